@@ -1,0 +1,229 @@
+"""Batched persistent scan: Python side of ``bgo_genome`` / ``bgo_plan``.
+
+One launch scans both strands of the whole genome (or a position range of
+it) for any mix of guides, PAMs, editors and mismatch budgets, and returns
+fully scored site records.  Sites are those of the reference extensive kernel
+(beguidedoff/cuda_search.py:163-314) for each guide; the records carry the
+finder's per-hit epilogue (beguidedoff/finder.py:435-470).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .base_editing import EDITABLE_BASE, edit_window_mask
+from .encoding import encode_dna, encode_guide_64bit
+from .pam import encode_pam_mask
+from .scoring import cctop_table
+
+HIT_DTYPE = _lib.HIT_DTYPE
+_EDITOR_CODE = {"A": 0, "C": 1}
+
+
+@dataclass(frozen=True)
+class GuideSpec:
+    """One guide of a batched scan.  Only the first 20 nt are compared and the
+    PAM is read right after them (the reference's 20-nt layout,
+    cuda_search.py:239-240)."""
+
+    seq: str
+    pam: str
+    max_mm: int
+    editor_type: str = "ABE"
+    window: Tuple[int, int] = (4, 8)
+
+
+def guide_descs(specs: Sequence[GuideSpec]) -> np.ndarray:
+    d = np.zeros(len(specs), dtype=_lib.GUIDE_DESC_DTYPE)
+    for i, s in enumerate(specs):
+        d["seq2bit"][i] = encode_guide_64bit(encode_dna(s.seq)) & ((1 << 40) - 1)
+        d["pam_mask"][i] = encode_pam_mask(s.pam)
+        d["pam_len"][i] = len(s.pam)
+        if not 0 <= int(s.max_mm) <= 20:
+            raise ValueError(f"max_mm must be in 0..20, got {s.max_mm}")
+        d["max_mm"][i] = int(s.max_mm)
+        target = EDITABLE_BASE.get(s.editor_type)
+        if target is None:
+            raise ValueError(f"Unknown editor_type: {s.editor_type}")
+        d["editor"][i] = _EDITOR_CODE[target]
+        d["edit_mask"][i] = edit_window_mask(*s.window)
+    return d
+
+
+def _tuning_struct(tuning: Optional[dict]) -> Optional[_lib.Tuning]:
+    if not tuning:
+        return None
+    t = _lib.Tuning()
+    for k, v in tuning.items():
+        if k == "reserved":
+            continue
+        setattr(t, k, int(v))
+    return t
+
+
+class DeviceGenome:
+    """Both strand arrays of a genome resident in HBM (bgo_genome)."""
+
+    def __init__(self, genome: dict, n_mask_fwd: Optional[np.ndarray] = None,
+                 n_mask_rc: Optional[np.ndarray] = None):
+        lib = _lib.load()
+        fwd = np.ascontiguousarray(genome["genome_forward"], dtype=np.uint32)
+        rc = genome.get("genome_rc")
+        rc = None if rc is None else np.ascontiguousarray(rc, dtype=np.uint32)
+        if rc is not None and rc.size != fwd.size:
+            raise ValueError("genome_rc and genome_forward differ in size")
+        self.total_bp = int(genome["total_bp"])
+        nbytes = 0
+        if n_mask_fwd is not None:
+            n_mask_fwd = np.ascontiguousarray(n_mask_fwd, dtype=np.uint8)
+            nbytes = n_mask_fwd.size
+            if rc is not None:
+                if n_mask_rc is None:
+                    raise ValueError("an N mask for the reverse-complement array is required")
+                n_mask_rc = np.ascontiguousarray(n_mask_rc, dtype=np.uint8)
+                if n_mask_rc.size != nbytes:
+                    raise ValueError("forward and reverse N masks differ in size")
+        h = C.c_void_p()
+        _lib.check(lib.bgo_genome_create(
+            _lib.ptr(fwd), _lib.ptr(rc) if rc is not None else C.c_void_p(0), fwd.size,
+            _lib.ptr(n_mask_fwd) if nbytes else C.c_void_p(0),
+            _lib.ptr(n_mask_rc) if (nbytes and rc is not None) else C.c_void_p(0), nbytes,
+            self.total_bp, C.byref(h)))
+        self._h = h
+        self.has_rc = rc is not None
+        self.has_n_mask = nbytes > 0
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if self._h is None:
+            raise RuntimeError("genome released")
+        return self._h
+
+    def device_bytes(self) -> int:
+        n = C.c_int64(0)
+        _lib.check(_lib.load().bgo_genome_device_bytes(self.handle, C.byref(n)))
+        return int(n.value)
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.load().bgo_genome_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """A prepared batched scan with its own HIP stream and hit buffer."""
+
+    def __init__(self, genome: DeviceGenome, specs: Sequence[GuideSpec],
+                 tuning: Optional[dict] = None, hit_capacity: int = 1 << 20):
+        lib = _lib.load()
+        self.genome = genome
+        self.specs = list(specs)
+        self.n_guides = len(self.specs)
+        self.descs = guide_descs(self.specs)
+        self.score_table = np.array(cctop_table(), dtype=np.float64)
+        self.hit_capacity = int(hit_capacity)
+        h = C.c_void_p()
+        t = _tuning_struct(tuning)
+        _lib.check(lib.bgo_plan_create(genome.handle, _lib.ptr(self.descs), self.n_guides,
+                                       _lib.ptr(self.score_table),
+                                       C.byref(t) if t is not None else None,
+                                       self.hit_capacity, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if self._h is None:
+            raise RuntimeError("plan released")
+        return self._h
+
+    def launch(self, pos_begin: int = 0, pos_end: Optional[int] = None, strands: int = 3) -> None:
+        end = self.genome.total_bp if pos_end is None else int(pos_end)
+        _lib.check(_lib.load().bgo_plan_launch(self.handle, int(pos_begin), end, int(strands)))
+
+    def graph_launch(self, strands: int = 3) -> None:
+        _lib.check(_lib.load().bgo_plan_graph_launch(self.handle, int(strands)))
+
+    def wait(self) -> Tuple[int, float]:
+        n = C.c_int64(0)
+        ms = C.c_float(0.0)
+        _lib.check(_lib.load().bgo_plan_wait(self.handle, C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
+
+    def fetch(self, n: Optional[int] = None) -> np.ndarray:
+        cap = self.hit_capacity if n is None else min(int(n), self.hit_capacity)
+        out = np.empty(max(cap, 0), dtype=HIT_DTYPE)
+        got = C.c_int64(0)
+        _lib.check(_lib.load().bgo_plan_fetch(self.handle, _lib.ptr(out), out.size,
+                                              C.byref(got)))
+        return out[: got.value]
+
+    def tuning(self) -> dict:
+        t = _lib.Tuning()
+        _lib.check(_lib.load().bgo_plan_get_tuning(self.handle, C.byref(t)))
+        return t.as_dict()
+
+    def family_stats(self) -> List[Tuple[int, int]]:
+        """[(guides, PAM-passing windows compared in the last launch)] per family."""
+        nf = C.c_int32(0)
+        _lib.check(_lib.load().bgo_plan_family_stats(self.handle, 0, C.byref(nf), None, None))
+        w = np.zeros(max(nf.value, 1), dtype=np.int64)
+        g = np.zeros(max(nf.value, 1), dtype=np.int32)
+        _lib.check(_lib.load().bgo_plan_family_stats(self.handle, nf.value, C.byref(nf),
+                                                     _lib.ptr(w), _lib.ptr(g)))
+        return [(int(g[i]), int(w[i])) for i in range(nf.value)]
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.load().bgo_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class ScanResult:
+    hits: np.ndarray                 # HIT_DTYPE records, unordered
+    n_hits: int
+    kernel_ms: float
+    tuning: dict
+    families: List[Tuple[int, int]] = field(default_factory=list)
+    launches: int = 1
+
+
+def default_capacity(n_guides: int) -> int:
+    return max(1 << 20, 4096 * n_guides)
+
+
+def scan(genome: DeviceGenome, specs: Sequence[GuideSpec], tuning: Optional[dict] = None,
+         hit_capacity: Optional[int] = None, pos_begin: int = 0,
+         pos_end: Optional[int] = None, strands: int = 3) -> ScanResult:
+    """Scan and return every site; re-runs once with an exact-size buffer when
+    the first buffer was too small (the kernel counts past its capacity)."""
+    cap = int(hit_capacity or default_capacity(len(specs)))
+    launches = 0
+    while True:
+        plan = Plan(genome, specs, tuning=tuning, hit_capacity=cap)
+        try:
+            plan.launch(pos_begin, pos_end, strands)
+            n, ms = plan.wait()
+            launches += 1
+            if n <= cap:
+                return ScanResult(plan.fetch(n), n, ms, plan.tuning(), plan.family_stats(),
+                                  launches)
+            cap = n
+        finally:
+            plan.close()

@@ -1,0 +1,815 @@
+// Batched persistent scan engine for gfx950.
+//
+// Computes the same sites as the reference extensive kernel
+// (beguidedoff/cuda_search.py:163-314) for any mix of guides, PAMs, editors
+// and mismatch budgets in one launch, and fuses the finder's per-hit
+// epilogue (beguidedoff/finder.py:435-470: 2-bit mismatch positions, CCTOP
+// score, editable-window test).
+//
+// Layout.  Each strand array is re-laid out once as bit planes: for every
+// 32-base block b, planes[b] = {P0, P1} where bit j of P0 (P1) is bit 0
+// (bit 1) of the 2-bit code of base 32b + j.  A 20-nt window then is two
+// 20-bit words, and a guide/window mismatch mask is (W0^g0)|(W1^g1) -- one
+// v_xor + one v_bitop3 -- followed by one v_bcnt.  The N mask is used as
+// uint32 words of the reference's little-endian bytes.
+//
+// Work decomposition.  Persistent waves pull work items (a strand and a run
+// of `chunk` positions) from an atomic counter.  A wave streams its run 64
+// positions at a time (one position per lane): it extracts the 26-base
+// window+PAM of both planes, evaluates the IUPAC PAM of every PAM family of
+// the launch with three bit-selects, and appends the passing windows to a
+// per-family list in LDS (ballot + mbcnt compaction, no atomics).  Whenever a
+// family list holds 64 windows, the wave compares them against all guides of
+// that family (guides are wave-uniform, read through the scalar cache):
+// 8 guides per step, min-reduced so the common "no site" case costs one
+// compare per 8 guides.  Candidates take the exact path: N correction,
+// per-guide budget, epilogue, and a wave-aggregated append of hit records.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "bgo_common.h"
+
+struct bgo_genome {
+  int device = 0;
+  int64_t total_bp = 0;
+  int64_t n_words = 0;
+  uint64_t n_blocks = 0;  // 32-base blocks, including zero padding
+  int n_strands = 0;
+  uint2* d_planes[2] = {nullptr, nullptr};
+  uint32_t* d_nwords[2] = {nullptr, nullptr};
+  int64_t bytes = 0;
+};
+
+namespace bgo {
+namespace bk {
+
+constexpr int kMaxFam = 8;     // PAM families per kernel launch
+constexpr int kList = 128;     // window slots per family per wave
+constexpr int kUnroll = 8;     // guides per compare step
+constexpr int kMaxBlock = 256; // threads per block (4 waves)
+constexpr uint32_t kWin = 0xFFFFFu;
+
+struct Fam {
+  uint32_t mA, mC, mG, mT;  // allowed-base masks at bits 20..25, all other bits 1
+  uint32_t g_begin, g_end;  // padded guide range
+  uint32_t kmax;            // largest budget among the family's guides
+  uint32_t pad;
+};
+
+struct Args {
+  const uint2* planes0;
+  const uint2* planes1;
+  const uint32_t* nwords0;
+  const uint32_t* nwords1;
+  uint32_t slot_strand0, slot_strand1;  // strand scanned by enumeration slot 0/1
+  uint32_t valid_begin, valid_end;
+  uint32_t chunk, chunks_per_slot, n_items, n_fam;
+  unsigned long long hit_cap;
+};
+
+// Read-only tables and outputs.  They reach the kernel as separate
+// __restrict__ parameters so wave-uniform reads of the guide and family
+// tables compile to scalar (s_load) loads.
+struct Tab {
+  const Fam* __restrict__ fam;           // [n_fam]
+  const uint32_t* __restrict__ g01;      // [2 * n_pad]: g0, g1 planes per guide
+  const int32_t* __restrict__ gk;        // [n_pad] budget, -1 for padding
+  const uint32_t* __restrict__ gaux;     // [n_pad] edit window mask | editor << 24
+  const uint32_t* __restrict__ gid;      // [n_pad] caller's guide index
+  const uint32_t* __restrict__ gnm;      // [2 * n_pad] guide positions equal to the N code
+  const double* __restrict__ score_tab;  // [21]
+  bgo_hit* __restrict__ hits;
+  unsigned long long* __restrict__ hit_count;
+};
+
+__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exact path for a (window, guide) pair whose 2-bit mismatch count c is within
+// the guide's budget: add the N correction (cuda_search.py:290-308), re-test,
+// and append a fully scored record.
+__device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s, uint32_t gi,
+                                         uint32_t pos, uint32_t W0, uint32_t W1, uint32_t c,
+                                         bool cand, uint32_t lane) {
+  const uint32_t g0 = t.g01[2 * gi], g1 = t.g01[2 * gi + 1];
+  const int k = t.gk[gi];
+  int nmm = static_cast<int>(c);
+  bool rec = false;
+  if (cand) {
+    const uint32_t* nw = s ? a.nwords1 : a.nwords0;
+    if (nw) {
+      const uint32_t w = pos >> 5, sh = pos & 31u;
+      const uint32_t nb = funnel(nw[w + 1], nw[w], sh) & kWin;
+      if (nb) nmm += __popc(nb & t.gnm[2 * gi + s]);
+    }
+    rec = nmm <= k;
+  }
+  const uint64_t bal = ballot(rec);
+  if (!bal) return;
+  const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(bal));
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(t.hit_count, static_cast<unsigned long long>(__popcll(bal)));
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(base), leader);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(base >> 32), leader);
+  base = (static_cast<unsigned long long>(hi) << 32) | lo;
+  if (!rec) return;
+  const unsigned long long idx = base + lanes_below(bal);
+  if (idx >= a.hit_cap) return;
+  const uint2* planes = s ? a.planes1 : a.planes0;
+  const uint32_t w = pos >> 5, sh = pos & 31u;
+  const uint2 qa = planes[w], qb = planes[w + 1];
+  bgo_hit h;
+  h.guide = t.gid[gi];
+  h.pos = pos;
+  h.x0 = funnel(qb.x, qa.x, sh) & 0x3FFFFFFu;
+  h.x1 = funnel(qb.y, qa.y, sh) & 0x3FFFFFFu;
+  uint32_t mm = ((W0 ^ g0) | (W1 ^ g1)) & kWin;
+  h.mm_mask = mm;
+  h.strand = static_cast<uint8_t>(s);
+  h.n_mm = static_cast<uint8_t>(nmm);
+  const uint32_t aux = t.gaux[gi];
+  const uint32_t amask = ~(W0 | W1) & kWin;  // window bases == A
+  const uint32_t cmask = W0 & ~W1 & kWin;    // window bases == C
+  h.editable = (((aux >> 24) & 1u ? cmask : amask) & aux & kWin) != 0;
+  h.pad = 0;
+  double sc = 0.0;  // ascending positions, as sum(1.2 ** p for p in positions)
+  while (mm) {
+    const int i = __builtin_ctz(mm);
+    mm &= mm - 1;
+    sc += t.score_tab[i + 1];
+  }
+  h.score = sc;
+  t.hits[idx] = h;
+}
+
+// Compare the first n (<= 64) windows of a family list with all guides of
+// the family.  One window per lane; guides are wave-uniform.
+__device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, uint32_t s,
+                                    const uint2* L, uint32_t n, uint32_t cs, uint32_t lane) {
+  if (lane >= n) return;
+  const uint2 e = L[lane];
+  const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
+  const uint32_t pos = cs + ((e.x >> 20) | ((e.y >> 20) << 12));
+  const uint32_t kmax = F.kmax;
+  const uint32_t* __restrict__ g01 = t.g01;
+  for (uint32_t g = F.g_begin; g < F.g_end; g += kUnroll) {
+    // 8 guides = 64 aligned bytes, one wide scalar load.
+    const uint4* __restrict__ gq = reinterpret_cast<const uint4*>(g01 + 2 * g);
+    const uint4 q0 = gq[0], q1 = gq[1], q2 = gq[2], q3 = gq[3];
+    uint32_t c[kUnroll];
+    c[0] = __popc((W0 ^ q0.x) | (W1 ^ q0.y));
+    c[1] = __popc((W0 ^ q0.z) | (W1 ^ q0.w));
+    c[2] = __popc((W0 ^ q1.x) | (W1 ^ q1.y));
+    c[3] = __popc((W0 ^ q1.z) | (W1 ^ q1.w));
+    c[4] = __popc((W0 ^ q2.x) | (W1 ^ q2.y));
+    c[5] = __popc((W0 ^ q2.z) | (W1 ^ q2.w));
+    c[6] = __popc((W0 ^ q3.x) | (W1 ^ q3.y));
+    c[7] = __popc((W0 ^ q3.z) | (W1 ^ q3.w));
+    uint32_t m = min(min(min(c[0], c[1]), min(c[2], c[3])), min(min(c[4], c[5]), min(c[6], c[7])));
+    if (ballot(m <= kmax)) {
+      // Rare: recompute per guide so the exact path exists once in the code.
+#pragma nounroll
+      for (uint32_t i = 0; i < kUnroll; ++i) {
+        const uint32_t gi = g + i;
+        const uint32_t ci = __popc((W0 ^ g01[2 * gi]) | (W1 ^ g01[2 * gi + 1]));
+        const bool cand = static_cast<int>(ci) <= t.gk[gi];
+        if (ballot(cand)) record_candidates(a, t, s, gi, pos, W0, W1, ci, cand, lane);
+      }
+    }
+  }
+}
+
+template <int NF>
+__device__ inline void scan_run(const Args& a, const Tab& t, uint32_t s, uint32_t cs, uint32_t ce,
+                                uint2* lists, unsigned long long* wstats, uint32_t lane) {
+  const uint2* __restrict__ planes = s ? a.planes1 : a.planes0;
+  constexpr uint32_t nf = NF;
+  // Family masks, wave-uniform for the whole run.
+  uint32_t mA[NF], mC[NF], mG[NF], mT[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    mA[f] = t.fam[f].mA;
+    mC[f] = t.fam[f].mC;
+    mG[f] = t.fam[f].mG;
+    mT[f] = t.fam[f].mT;
+  }
+  uint64_t cnts = 0;  // 8-bit fill level per family list
+  for (uint32_t base = cs; base < ce; base += 64) {
+    const uint32_t p = base + lane;
+    const bool valid = p < ce;
+    const uint32_t pp = valid ? p : base;
+    const uint32_t w = pp >> 5, sh = pp & 31u;
+    const uint2 qa = planes[w], qb = planes[w + 1];
+    const uint32_t X0 = funnel(qb.x, qa.x, sh);
+    const uint32_t X1 = funnel(qb.y, qa.y, sh);
+    const uint32_t off = pp - cs;
+    const uint2 entry = make_uint2((X0 & kWin) | (off << 20), (X1 & kWin) | ((off >> 12) << 20));
+    bool full = false;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      // IUPAC test of the 6 PAM bases at once: select the allowed bit of
+      // each base's code from the four per-base masks.
+      const uint32_t lo = (X0 & mC[f]) | (~X0 & mA[f]);
+      const uint32_t hi = (X0 & mT[f]) | (~X0 & mG[f]);
+      const uint32_t Y = (X1 & hi) | (~X1 & lo);
+      const bool pass = valid && (Y == 0xFFFFFFFFu);
+      const uint64_t bal = ballot(pass);
+      if (bal) {
+        uint32_t c = static_cast<uint32_t>(cnts >> (8 * f)) & 0xFFu;
+        if (pass) lists[f * kList + c + lanes_below(bal)] = entry;
+        c += static_cast<uint32_t>(__popcll(bal));
+        cnts = (cnts & ~(0xFFull << (8 * f))) | (static_cast<uint64_t>(c) << (8 * f));
+        full |= (c >= 64);
+      }
+    }
+    if (full) {
+      wave_sync();
+      for (uint32_t f = 0; f < nf; ++f) {
+        uint32_t c = static_cast<uint32_t>(cnts >> (8 * f)) & 0xFFu;
+        if (c < 64) continue;
+        uint2* L = lists + f * kList;
+        compare_list(a, t, t.fam[f], s, L, 64, cs, lane);
+        wave_sync();
+        if (lane < c - 64) L[lane] = L[64 + lane];
+        wave_sync();
+        c -= 64;
+        cnts = (cnts & ~(0xFFull << (8 * f))) | (static_cast<uint64_t>(c) << (8 * f));
+        if (lane == 0) wstats[f] += 64;
+      }
+    }
+  }
+  wave_sync();
+  for (uint32_t f = 0; f < nf; ++f) {
+    const uint32_t c = static_cast<uint32_t>(cnts >> (8 * f)) & 0xFFu;
+    if (!c) continue;
+    compare_list(a, t, t.fam[f], s, lists + f * kList, c, cs, lane);
+    if (lane == 0) wstats[f] += c;
+  }
+  wave_sync();
+}
+
+template <int NF>
+__global__ __launch_bounds__(kMaxBlock) void batched_valu_kernel(
+    Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
+    const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
+    const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gnm,
+    const double* __restrict__ score_tab, bgo_hit* __restrict__ hits,
+    unsigned long long* __restrict__ hit_count, unsigned int* __restrict__ work_counter,
+    unsigned long long* __restrict__ fam_windows) {
+  const Tab t{fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count};
+  extern __shared__ uint2 lds[];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t n_waves = blockDim.x >> 6;
+  const uint32_t per_wave = NF * kList;
+  uint2* lists = lds + wave * per_wave;
+  auto* wstats = reinterpret_cast<unsigned long long*>(lds + n_waves * per_wave) + wave * kMaxFam;
+  if (lane < kMaxFam) wstats[lane] = 0;
+  wave_sync();
+  for (;;) {
+    uint32_t item = 0;
+    if (lane == 0) item = atomicAdd(work_counter, 1u);
+    item = __builtin_amdgcn_readlane(item, 0);
+    if (item >= a.n_items) break;
+    const uint32_t slot = item / a.chunks_per_slot;
+    const uint32_t s = slot ? a.slot_strand1 : a.slot_strand0;
+    const uint32_t cs = a.valid_begin + (item - slot * a.chunks_per_slot) * a.chunk;
+    const uint32_t ce = min(cs + a.chunk, a.valid_end);
+    scan_run<NF>(a, t, s, cs, ce, lists, wstats, lane);
+  }
+  wave_sync();
+  if (lane < NF && wstats[lane]) atomicAdd(&fam_windows[lane], wstats[lane]);
+}
+
+using KernelFn = void (*)(Args, const Fam*, const uint32_t*, const int32_t*, const uint32_t*,
+                         const uint32_t*, const uint32_t*, const double*, bgo_hit*,
+                         unsigned long long*, unsigned int*, unsigned long long*);
+
+inline KernelFn valu_kernel_for(int nf) {
+  switch (nf) {
+    case 1: return batched_valu_kernel<1>;
+    case 2: return batched_valu_kernel<2>;
+    case 3: return batched_valu_kernel<3>;
+    case 4: return batched_valu_kernel<4>;
+    case 5: return batched_valu_kernel<5>;
+    case 6: return batched_valu_kernel<6>;
+    case 7: return batched_valu_kernel<7>;
+    default: return batched_valu_kernel<8>;
+  }
+}
+
+__global__ void reset_counters_kernel(unsigned long long* c, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
+}
+
+// 2-bit words -> bit planes, one 32-base block per lane.
+__global__ void planes_kernel(const uint32_t* __restrict__ words, uint64_t n_words,
+                              uint2* __restrict__ planes, uint64_t n_blocks) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= n_blocks) return;
+  const uint32_t w0 = (2 * b < n_words) ? words[2 * b] : 0u;
+  const uint32_t w1 = (2 * b + 1 < n_words) ? words[2 * b + 1] : 0u;
+  planes[b] = make_uint2(even_bits(w0) | (even_bits(w1) << 16),
+                         even_bits(w0 >> 1) | (even_bits(w1 >> 1) << 16));
+}
+
+}  // namespace bk
+}  // namespace bgo
+
+using namespace bgo;
+using bgo::bk::Fam;
+
+struct bgo_plan {
+  bgo_genome* g = nullptr;
+  int device = 0;
+  int32_t n_guides = 0;
+  int64_t hit_cap = 0;
+  bgo_tuning tune{};
+  // families, grouped into launches of <= kMaxFam
+  std::vector<Fam> fams;              // all families (g ranges into the padded table)
+  std::vector<int32_t> fam_guides;    // real guides per family
+  std::vector<int> group_first;       // first family of each launch group
+  // device memory
+  void* d_mem = nullptr;
+  Fam* d_fam = nullptr;
+  uint32_t* d_g01 = nullptr;
+  int32_t* d_gk = nullptr;
+  uint32_t* d_gaux = nullptr;
+  uint32_t* d_gid = nullptr;
+  uint32_t* d_gnm = nullptr;
+  double* d_score = nullptr;
+  unsigned long long* d_counters = nullptr;  // [1 hit count][groups work counters][fam windows]
+  uint32_t n_counter_words = 0;
+  bgo_hit* d_hits = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_mask = -1;
+  std::vector<int> group_blocks;     // persistent blocks per launch group
+  std::vector<size_t> group_lds;     // dynamic LDS bytes per launch group
+  int threads = 0;
+  std::mutex mu;
+};
+
+namespace {
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+int build_genome_strand(bgo_genome* G, int s, const uint32_t* words, const uint8_t* nmask,
+                        int64_t nmask_bytes) {
+  uint32_t* d_words = nullptr;
+  BGO_HIP(hipMalloc(&d_words, G->n_words * sizeof(uint32_t)));
+  if (hipMemcpy(d_words, words, G->n_words * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    (void)hipFree(d_words);
+    return fail(BGO_ERR_HIP, "upload genome words");
+  }
+  int rc = BGO_OK;
+  if (hipMalloc(&G->d_planes[s], G->n_blocks * sizeof(uint2)) != hipSuccess) {
+    rc = fail(BGO_ERR_HIP, "hipMalloc planes");
+  } else {
+    const uint64_t grid = (G->n_blocks + 255) / 256;
+    hipLaunchKernelGGL(bk::planes_kernel, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, 0,
+                       d_words, static_cast<uint64_t>(G->n_words), G->d_planes[s], G->n_blocks);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+      rc = fail(BGO_ERR_HIP, "planes kernel");
+  }
+  (void)hipFree(d_words);
+  if (rc != BGO_OK) return rc;
+  G->bytes += G->n_blocks * sizeof(uint2);
+  if (nmask && nmask_bytes > 0) {
+    const size_t nbytes = G->n_blocks * sizeof(uint32_t);
+    BGO_HIP(hipMalloc(&G->d_nwords[s], nbytes));
+    BGO_HIP(hipMemset(G->d_nwords[s], 0, nbytes));
+    BGO_HIP(hipMemcpy(G->d_nwords[s], nmask, std::min<size_t>(nmask_bytes, nbytes),
+                      hipMemcpyHostToDevice));
+    G->bytes += nbytes;
+  }
+  return BGO_OK;
+}
+
+}  // namespace
+
+extern "C" int bgo_genome_create(const uint32_t* words_fwd, const uint32_t* words_rc,
+                                 int64_t n_words, const uint8_t* nmask_fwd,
+                                 const uint8_t* nmask_rc, int64_t nmask_bytes, int64_t total_bp,
+                                 bgo_genome** out) {
+  if (!out || !words_fwd || n_words <= 0 || total_bp < 0 || total_bp > n_words * 16)
+    return fail(BGO_ERR_ARG, "bgo_genome_create: bad arguments");
+  if (total_bp >= (int64_t(1) << 32) - 64)
+    return fail(BGO_ERR_ARG, "bgo_genome_create: total_bp must be < 2^32 - 64");
+  if (nmask_bytes < 0 || (nmask_bytes > 0 && !nmask_fwd) ||
+      (nmask_bytes > 0 && words_rc && !nmask_rc))
+    return fail(BGO_ERR_ARG, "bgo_genome_create: bad N mask");
+  BGO_TRY(injected());
+  auto* G = new bgo_genome();
+  BGO_HIP(hipGetDevice(&G->device));
+  G->total_bp = total_bp;
+  G->n_words = n_words;
+  // +4 zero blocks: the kernel reads planes[w + 1] for w <= (total_bp + 63) / 32.
+  G->n_blocks = static_cast<uint64_t>((n_words + 1) / 2) + 4;
+  G->n_strands = words_rc ? 2 : 1;
+  int rc = build_genome_strand(G, 0, words_fwd, nmask_fwd, nmask_bytes);
+  if (rc == BGO_OK && words_rc) rc = build_genome_strand(G, 1, words_rc, nmask_rc, nmask_bytes);
+  if (rc != BGO_OK) {
+    bgo_genome_destroy(G);
+    return rc;
+  }
+  *out = G;
+  return BGO_OK;
+}
+
+extern "C" int bgo_genome_destroy(bgo_genome* G) {
+  if (!G) return BGO_OK;
+  for (int s = 0; s < 2; ++s) {
+    if (G->d_planes[s]) (void)hipFree(G->d_planes[s]);
+    if (G->d_nwords[s]) (void)hipFree(G->d_nwords[s]);
+  }
+  delete G;
+  return BGO_OK;
+}
+
+extern "C" int bgo_genome_device_bytes(const bgo_genome* G, int64_t* bytes_out) {
+  if (!G || !bytes_out) return fail(BGO_ERR_ARG, "null pointer");
+  *bytes_out = G->bytes;
+  return BGO_OK;
+}
+
+namespace {
+
+// Family key: PAM length + allowed-base nibbles.
+uint64_t fam_key(const bgo_guide_desc& d) {
+  uint64_t k = d.pam_len;
+  for (int i = 0; i < d.pam_len && i < 6; ++i) k |= static_cast<uint64_t>(d.pam_mask[i] & 0xF) << (4 + 4 * i);
+  return k;
+}
+
+Fam make_fam(const bgo_guide_desc& d) {
+  Fam F{};
+  uint32_t a = 0, c = 0, gg = 0, t = 0, pam_bits = 0;
+  for (int i = 0; i < d.pam_len; ++i) {
+    const uint32_t nib = d.pam_mask[i] & 0xFu;
+    const uint32_t bit = 1u << (20 + i);
+    pam_bits |= bit;
+    if (nib & 1u) a |= bit;
+    if (nib & 2u) c |= bit;
+    if (nib & 4u) gg |= bit;
+    if (nib & 8u) t |= bit;
+  }
+  const uint32_t rest = ~pam_bits;
+  F.mA = a | rest;
+  F.mC = c | rest;
+  F.mG = gg | rest;
+  F.mT = t | rest;
+  return F;
+}
+
+int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
+  bgo_tuning r{};
+  if (t) r = *t;
+  if (r.chunk_positions <= 0) r.chunk_positions = 131072;
+  r.chunk_positions = std::max(64, (r.chunk_positions + 63) / 64 * 64);
+  if (r.chunk_positions > (1 << 24)) return fail(BGO_ERR_ARG, "chunk_positions must be <= 2^24");
+  if (r.waves_per_block <= 0) r.waves_per_block = 4;
+  if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
+    return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");
+  if (r.engine <= 0) r.engine = 1;
+  if (r.engine != 1) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
+  r.guide_unroll = bk::kUnroll;
+  P->tune = r;
+  return BGO_OK;
+}
+
+int plan_configure_launch(bgo_plan* P) {
+  int n_cu = 0;
+  BGO_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, P->device));
+  P->threads = P->tune.waves_per_block * 64;
+  int min_occ = 1 << 30;
+  for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
+    const int nf = std::min<int>(bk::kMaxFam, static_cast<int>(P->fams.size()) - P->group_first[gi]);
+    const size_t per_wave = static_cast<size_t>(nf) * bk::kList * sizeof(uint2) +
+                            bk::kMaxFam * sizeof(unsigned long long);
+    const size_t lds = per_wave * P->tune.waves_per_block;
+    int occ = 0;
+    BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bk::valu_kernel_for(nf), P->threads,
+                                                         lds));
+    if (occ <= 0) occ = 1;
+    if (P->tune.blocks_per_cu > 0) occ = std::min(occ, P->tune.blocks_per_cu);
+    min_occ = std::min(min_occ, occ);
+    P->group_blocks.push_back(occ * n_cu);
+    P->group_lds.push_back(lds);
+  }
+  P->tune.blocks_per_cu = P->group_first.empty() ? 0 : min_occ;
+  return BGO_OK;
+}
+
+}  // namespace
+
+extern "C" int bgo_plan_destroy(bgo_plan* P) {
+  if (!P) return BGO_OK;
+  if (P->graph) (void)hipGraphExecDestroy(P->graph);
+  if (P->d_mem) (void)hipFree(P->d_mem);
+  if (P->d_hits) (void)hipFree(P->d_hits);
+  if (P->ev0) (void)hipEventDestroy(P->ev0);
+  if (P->ev1) (void)hipEventDestroy(P->ev1);
+  if (P->stream) (void)hipStreamDestroy(P->stream);
+  delete P;
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int32_t n_guides,
+                               const double* score_table, const bgo_tuning* tuning,
+                               int64_t hit_capacity, bgo_plan** out) {
+  if (!G || !out || n_guides < 0 || (n_guides > 0 && !guides) || !score_table ||
+      hit_capacity < 0)
+    return fail(BGO_ERR_ARG, "bgo_plan_create: bad arguments");
+  for (int32_t i = 0; i < n_guides; ++i) {
+    const auto& d = guides[i];
+    if (d.pam_len < 1 || d.pam_len > BGO_MAX_PAM_LEN)
+      return fail(BGO_ERR_ARG, "guide %d: pam_len %d not in 1..6", i, d.pam_len);
+    if (d.max_mm > 20) return fail(BGO_ERR_ARG, "guide %d: max_mm %d > 20", i, d.max_mm);
+    if (d.editor > 1) return fail(BGO_ERR_ARG, "guide %d: editor %d", i, d.editor);
+  }
+  BGO_TRY(injected());
+  auto* P = new bgo_plan();
+  auto bail = [&](int rc) {
+    bgo_plan_destroy(P);
+    return rc;
+  };
+  P->g = G;
+  P->device = G->device;
+  P->n_guides = n_guides;
+  P->hit_cap = hit_capacity;
+  if (hipSetDevice(G->device) != hipSuccess) return bail(fail(BGO_ERR_HIP, "hipSetDevice"));
+  if (int rc = resolve_tuning(P, tuning); rc != BGO_OK) return bail(rc);
+
+  // Group guides by PAM family, in order of first appearance.
+  std::map<uint64_t, int> fam_index;
+  std::vector<std::vector<int32_t>> members;
+  std::vector<Fam> proto;
+  for (int32_t i = 0; i < n_guides; ++i) {
+    const uint64_t key = fam_key(guides[i]);
+    auto it = fam_index.find(key);
+    if (it == fam_index.end()) {
+      fam_index.emplace(key, static_cast<int>(members.size()));
+      members.emplace_back();
+      proto.push_back(make_fam(guides[i]));
+      it = fam_index.find(key);
+    }
+    members[it->second].push_back(i);
+  }
+  const int n_fam = static_cast<int>(members.size());
+  std::vector<uint32_t> g01, gaux, gid, gnm;
+  std::vector<int32_t> gk;
+  for (int f = 0; f < n_fam; ++f) {
+    Fam F = proto[f];
+    F.g_begin = static_cast<uint32_t>(gk.size());
+    uint32_t kmax = 0;
+    for (int32_t i : members[f]) {
+      const auto& d = guides[i];
+      uint32_t p0 = 0, p1 = 0, eqA = 0, eqT = 0;
+      for (int b = 0; b < 20; ++b) {
+        const uint32_t code = static_cast<uint32_t>(d.seq2bit >> (2 * b)) & 3u;
+        p0 |= (code & 1u) << b;
+        p1 |= (code >> 1) << b;
+        if (code == 0) eqA |= 1u << b;
+        if (code == 3) eqT |= 1u << b;
+      }
+      g01.push_back(p0);
+      g01.push_back(p1);
+      gk.push_back(d.max_mm);
+      gaux.push_back((d.edit_mask & bk::kWin) | (static_cast<uint32_t>(d.editor & 1) << 24));
+      gid.push_back(static_cast<uint32_t>(i));
+      gnm.push_back(eqA);
+      gnm.push_back(eqT);
+      kmax = std::max<uint32_t>(kmax, d.max_mm);
+    }
+    while ((gk.size() - F.g_begin) % bk::kUnroll) {  // padding guides never match
+      g01.push_back(0xFFF00000u);
+      g01.push_back(0xFFF00000u);
+      gk.push_back(-1);
+      gaux.push_back(0);
+      gid.push_back(0xFFFFFFFFu);
+      gnm.push_back(0);
+      gnm.push_back(0);
+    }
+    F.g_end = static_cast<uint32_t>(gk.size());
+    F.kmax = kmax;
+    P->fams.push_back(F);
+    P->fam_guides.push_back(static_cast<int32_t>(members[f].size()));
+  }
+  for (int f = 0; f < n_fam; f += bk::kMaxFam) P->group_first.push_back(f);
+  const size_t n_pad = gk.size();
+
+  // One device allocation for the tables and counters.
+  P->n_counter_words = 1 + static_cast<uint32_t>(P->group_first.size()) + n_fam;
+  size_t off = 0;
+  const size_t o_fam = off; off = align256(off + std::max<size_t>(n_fam, 1) * sizeof(Fam));
+  const size_t o_g01 = off; off = align256(off + std::max<size_t>(n_pad, 1) * 8);
+  const size_t o_gk = off; off = align256(off + std::max<size_t>(n_pad, 1) * 4);
+  const size_t o_gaux = off; off = align256(off + std::max<size_t>(n_pad, 1) * 4);
+  const size_t o_gid = off; off = align256(off + std::max<size_t>(n_pad, 1) * 4);
+  const size_t o_gnm = off; off = align256(off + std::max<size_t>(n_pad, 1) * 8);
+  const size_t o_score = off; off = align256(off + 21 * sizeof(double));
+  const size_t o_cnt = off; off = align256(off + P->n_counter_words * 8);
+  if (hipMalloc(&P->d_mem, off) != hipSuccess) return bail(fail(BGO_ERR_HIP, "hipMalloc plan"));
+  char* b = static_cast<char*>(P->d_mem);
+  P->d_fam = reinterpret_cast<Fam*>(b + o_fam);
+  P->d_g01 = reinterpret_cast<uint32_t*>(b + o_g01);
+  P->d_gk = reinterpret_cast<int32_t*>(b + o_gk);
+  P->d_gaux = reinterpret_cast<uint32_t*>(b + o_gaux);
+  P->d_gid = reinterpret_cast<uint32_t*>(b + o_gid);
+  P->d_gnm = reinterpret_cast<uint32_t*>(b + o_gnm);
+  P->d_score = reinterpret_cast<double*>(b + o_score);
+  P->d_counters = reinterpret_cast<unsigned long long*>(b + o_cnt);
+  bool ok = true;
+  if (n_fam) ok &= hipMemcpy(P->d_fam, P->fams.data(), n_fam * sizeof(Fam), hipMemcpyHostToDevice) == hipSuccess;
+  if (n_pad) {
+    ok &= hipMemcpy(P->d_g01, g01.data(), n_pad * 8, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(P->d_gk, gk.data(), n_pad * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(P->d_gaux, gaux.data(), n_pad * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(P->d_gid, gid.data(), n_pad * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(P->d_gnm, gnm.data(), n_pad * 8, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  ok &= hipMemcpy(P->d_score, score_table, 21 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+  ok &= hipMemset(P->d_counters, 0, P->n_counter_words * 8) == hipSuccess;
+  if (!ok) return bail(fail(BGO_ERR_HIP, "upload plan tables"));
+  if (hit_capacity > 0 && hipMalloc(&P->d_hits, hit_capacity * sizeof(bgo_hit)) != hipSuccess)
+    return bail(fail(BGO_ERR_HIP, "hipMalloc hit buffer (%lld records)", (long long)hit_capacity));
+  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&P->ev0) != hipSuccess || hipEventCreate(&P->ev1) != hipSuccess)
+    return bail(fail(BGO_ERR_HIP, "stream/event create"));
+  if (int rc = plan_configure_launch(P); rc != BGO_OK) return bail(rc);
+  *out = P;
+  return BGO_OK;
+}
+
+namespace {
+
+// Enqueue counter reset + one kernel per family group on P->stream.
+int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand_mask) {
+  bgo_genome* G = P->g;
+  hipLaunchKernelGGL(bk::reset_counters_kernel, dim3(1), dim3(256), 0, P->stream, P->d_counters,
+                     P->n_counter_words);
+  BGO_HIP(hipGetLastError());
+  const int64_t hard_end = G->total_bp - 26;  // cuda_search.py:211-213
+  const int64_t vb = std::max<int64_t>(pos_begin, 0);
+  const int64_t ve = std::min<int64_t>(pos_end, hard_end);
+  uint32_t slots[2];
+  uint32_t n_slots = 0;
+  if (strand_mask & 1) slots[n_slots++] = 0;
+  if ((strand_mask & 2) && G->n_strands > 1) slots[n_slots++] = 1;
+  if (ve <= vb || n_slots == 0 || P->fams.empty()) return BGO_OK;
+  const uint32_t chunk = static_cast<uint32_t>(P->tune.chunk_positions);
+  const uint64_t cps = (static_cast<uint64_t>(ve - vb) + chunk - 1) / chunk;
+  if (cps * n_slots >= 0xFFFFFFF0ull) return fail(BGO_ERR_ARG, "too many work items");
+  for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
+    const int f0 = P->group_first[gi];
+    const int nf = std::min<int>(bk::kMaxFam, static_cast<int>(P->fams.size()) - f0);
+    bk::Args a{};
+    a.planes0 = G->d_planes[0];
+    a.planes1 = G->d_planes[1];
+    a.nwords0 = G->d_nwords[0];
+    a.nwords1 = G->d_nwords[1];
+    a.slot_strand0 = slots[0];
+    a.slot_strand1 = n_slots > 1 ? slots[1] : slots[0];
+    a.valid_begin = static_cast<uint32_t>(vb);
+    a.valid_end = static_cast<uint32_t>(ve);
+    a.chunk = chunk;
+    a.chunks_per_slot = static_cast<uint32_t>(cps);
+    a.n_items = static_cast<uint32_t>(cps * n_slots);
+    a.n_fam = static_cast<uint32_t>(nf);
+    a.hit_cap = static_cast<unsigned long long>(P->hit_cap);
+    hipLaunchKernelGGL(bk::valu_kernel_for(nf), dim3(P->group_blocks[gi]), dim3(P->threads),
+                       P->group_lds[gi], P->stream, a, P->d_fam + f0, P->d_g01, P->d_gk, P->d_gaux, P->d_gid,
+                       P->d_gnm, P->d_score, P->d_hits, P->d_counters,
+                       reinterpret_cast<unsigned int*>(P->d_counters + 1 + gi),
+                       P->d_counters + 1 + P->group_first.size() + f0);
+    BGO_HIP(hipGetLastError());
+  }
+  return BGO_OK;
+}
+
+}  // namespace
+
+extern "C" int bgo_plan_launch(bgo_plan* P, int64_t pos_begin, int64_t pos_end,
+                               int32_t strand_mask) {
+  if (!P) return fail(BGO_ERR_ARG, "null plan");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipEventRecord(P->ev0, P->stream));
+  BGO_TRY(enqueue_scan(P, pos_begin, pos_end, strand_mask));
+  BGO_HIP(hipEventRecord(P->ev1, P->stream));
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_graph_launch(bgo_plan* P, int32_t strand_mask) {
+  if (!P) return fail(BGO_ERR_ARG, "null plan");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  if (!P->graph || P->graph_mask != strand_mask) {
+    if (P->graph) {
+      (void)hipGraphExecDestroy(P->graph);
+      P->graph = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    BGO_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_scan(P, 0, P->g->total_bp, strand_mask);
+    const hipError_t ce = hipStreamEndCapture(P->stream, &graph);
+    if (rc != BGO_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ce != hipSuccess) return fail(BGO_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&P->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(BGO_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    P->graph_mask = strand_mask;
+  }
+  BGO_HIP(hipEventRecord(P->ev0, P->stream));
+  BGO_HIP(hipGraphLaunch(P->graph, P->stream));
+  BGO_HIP(hipEventRecord(P->ev1, P->stream));
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_wait(bgo_plan* P, int64_t* n_hits_out, float* kernel_ms_out) {
+  if (!P || !n_hits_out) return fail(BGO_ERR_ARG, "null pointer");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  unsigned long long n = 0;
+  BGO_HIP(hipMemcpy(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost));
+  *n_hits_out = static_cast<int64_t>(n);
+  if (kernel_ms_out) {
+    float ms = 0.f;
+    BGO_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+    *kernel_ms_out = ms;
+  }
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_fetch(bgo_plan* P, bgo_hit* out, int64_t cap, int64_t* n_copied) {
+  if (!P || !n_copied || (cap > 0 && !out)) return fail(BGO_ERR_ARG, "bad arguments");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  unsigned long long n = 0;
+  BGO_HIP(hipMemcpy(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost));
+  const int64_t m = std::min<int64_t>({static_cast<int64_t>(n), P->hit_cap, cap});
+  if (m > 0) BGO_HIP(hipMemcpy(out, P->d_hits, m * sizeof(bgo_hit), hipMemcpyDeviceToHost));
+  *n_copied = std::max<int64_t>(m, 0);
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_get_tuning(const bgo_plan* P, bgo_tuning* out) {
+  if (!P || !out) return fail(BGO_ERR_ARG, "null pointer");
+  *out = P->tune;
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_family_stats(const bgo_plan* P, int32_t cap, int32_t* n_fam_out,
+                                     int64_t* windows_out, int32_t* guides_out) {
+  if (!P || !n_fam_out) return fail(BGO_ERR_ARG, "null pointer");
+  const int n_fam = static_cast<int>(P->fams.size());
+  *n_fam_out = n_fam;
+  if (cap <= 0) return BGO_OK;
+  BGO_TRY(injected());
+  std::vector<unsigned long long> w(std::max(n_fam, 1));
+  if (n_fam) {
+    BGO_HIP(hipSetDevice(P->device));
+    BGO_HIP(hipMemcpy(w.data(), P->d_counters + 1 + P->group_first.size(),
+                      n_fam * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  }
+  for (int f = 0; f < std::min(cap, n_fam); ++f) {
+    if (windows_out) windows_out[f] = static_cast<int64_t>(w[f]);
+    if (guides_out) guides_out[f] = P->fam_guides[f];
+  }
+  return BGO_OK;
+}
+
+extern "C" int bgo_scan_batched(bgo_genome* G, const bgo_guide_desc* guides, int32_t n_guides,
+                                const double* score_table, const bgo_tuning* tuning, bgo_hit* out,
+                                int64_t out_capacity, int64_t* n_hits_out) {
+  if (!n_hits_out || out_capacity < 0 || (out_capacity > 0 && !out))
+    return fail(BGO_ERR_ARG, "bgo_scan_batched: bad arguments");
+  bgo_plan* P = nullptr;
+  BGO_TRY(bgo_plan_create(G, guides, n_guides, score_table, tuning, out_capacity, &P));
+  int rc = bgo_plan_launch(P, 0, G->total_bp, 3);
+  int64_t n = 0, copied = 0;
+  if (rc == BGO_OK) rc = bgo_plan_wait(P, &n, nullptr);
+  if (rc == BGO_OK) rc = bgo_plan_fetch(P, out, out_capacity, &copied);
+  bgo_plan_destroy(P);
+  if (rc == BGO_OK) *n_hits_out = n;
+  return rc;
+}

@@ -1,0 +1,94 @@
+// Shared host/device helpers of libbgo_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "bgo.h"
+
+namespace bgo {
+
+// ----------------------------------------------------------------- errors
+
+inline std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+
+inline int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  last_error() = buf;
+  return code;
+}
+
+inline std::atomic<int>& fail_countdown() {
+  static std::atomic<int> n{0};
+  return n;
+}
+
+// Every entry point that touches the GPU calls this first; it turns an armed
+// bgo_debug_fail_next() into an error before any device work happens.
+inline int injected() {
+  int v = fail_countdown().load();
+  while (v > 0) {
+    if (fail_countdown().compare_exchange_weak(v, v - 1)) {
+      if (v == 1) return fail(BGO_ERR_INJECTED, "injected failure (bgo_debug_fail_next)");
+      return BGO_OK;
+    }
+  }
+  return BGO_OK;
+}
+
+#define BGO_HIP(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return ::bgo::fail(BGO_ERR_HIP, "%s failed: %s (%s:%d)", #expr,        \
+                         hipGetErrorString(e_), __FILE__, __LINE__);         \
+  } while (0)
+
+#define BGO_TRY(expr)              \
+  do {                             \
+    int rc_ = (expr);              \
+    if (rc_ != BGO_OK) return rc_; \
+  } while (0)
+
+// ------------------------------------------------------------ device math
+
+// Gather bits 0, 2, 4, ..., 30 of x into bits 0..15.
+__host__ __device__ inline uint32_t even_bits(uint32_t x) {
+  x &= 0x55555555u;
+  x = (x | (x >> 1)) & 0x33333333u;
+  x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+  x = (x | (x >> 4)) & 0x00FF00FFu;
+  x = (x | (x >> 8)) & 0x0000FFFFu;
+  return x;
+}
+
+// Bits [sh, sh + 32) of the 64-bit value hi:lo (sh in 0..31).
+__device__ inline uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t sh) {
+  return __builtin_amdgcn_alignbit(hi, lo, sh);
+}
+
+__device__ inline uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+}  // namespace bgo

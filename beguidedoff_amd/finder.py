@@ -1,0 +1,628 @@
+"""Extensive-mode off-target finder on MI355X.
+
+Drop-in for the reference's ``beguidedoff.finder.beguidedoff``
+(beguidedoff/finder.py:256-679): same constructor and ``search`` arguments,
+same deduplication, on-target exclusion, coordinate conventions, escalation
+and truncation policy, sort order and 12 output columns.
+
+Two scan engines produce the same hits:
+
+* ``engine="plain"`` runs the reference's control flow (strand loop, batches
+  of <= MAX_GUIDES keys, per-guide result buffers, overflow re-scans) over the
+  drop-in kernel behind :func:`beguidedoff_amd.cuda_search.search_genome_cuda`.
+* ``engine="batched"`` (default) scans both strands for every key in one
+  persistent launch of the batched kernel, which also returns the mismatch
+  mask and CCTOP score of each site.
+
+Per-hit decoding (coordinates, sequences, scores, editability, sorting) is
+vectorised with numpy and reproduces the reference's per-hit Python loop
+(finder.py:193-244, 405-472, 545-566) value for value.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import logging
+from collections import defaultdict
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import pandas as pd
+
+from . import _lib
+from .base_editing import EDITABLE_BASE, edit_window_mask
+from .encoding import encode_dna, encode_guide_64bit
+from .genome_converter import load_genome
+from .io import OUTPUT_COLUMNS, load_guides_from_tsv
+from .pam import encode_pam_mask
+from .scoring import cctop_table
+from .synth import nmask_le
+from .types import GuideInput
+
+_LOGGER = logging.getLogger("beguidedoff")
+
+_MAX_GUIDES_PER_LAUNCH = 2048          # cuda_search.MAX_GUIDES
+_DEFAULT_RESULTS_PER_GUIDE = 10240     # finder.py:103
+_MAX_RESULTS_PER_GUIDE = 2_621_440     # finder.py:111 (read at call time)
+_DECODE_DICT_BUDGET = 4_000_000        # finder.py:116
+_PROFILES_TSV = Path(__file__).resolve().parent / "base_editor_profiles.tsv"
+_W40 = (1 << 40) - 1
+
+
+@dataclass(frozen=True)
+class _ScanGuide:
+    """Deduplicated scan key (beguidedoff/finder.py:77-85)."""
+
+    guide_seq: str
+    pam: str
+    on_target_chrom_idx: Optional[int] = None
+    on_target_local_pos: Optional[int] = None
+    on_target_strand: Optional[str] = None
+
+
+class RawHits:
+    """Raw sites of one scan key: scanned strand (0 '+', 1 '-'), position in
+    the scanned strand array, N-corrected mismatch count, and (batched engine)
+    the fused-epilogue mismatch mask and score."""
+
+    __slots__ = ("strand", "pos", "n_mm", "mm_mask", "score")
+
+    def __init__(self, strand, pos, n_mm, mm_mask=None, score=None):
+        self.strand = np.asarray(strand, dtype=np.int8)
+        self.pos = np.asarray(pos, dtype=np.int64)
+        self.n_mm = np.asarray(n_mm, dtype=np.int64)
+        self.mm_mask = None if mm_mask is None else np.asarray(mm_mask, dtype=np.uint32)
+        self.score = None if score is None else np.asarray(score, dtype=np.float64)
+
+    def __len__(self) -> int:
+        return int(self.pos.size)
+
+    @staticmethod
+    def empty() -> "RawHits":
+        return RawHits(np.zeros(0), np.zeros(0), np.zeros(0))
+
+    @staticmethod
+    def concat(parts: Sequence["RawHits"]) -> "RawHits":
+        parts = [p for p in parts if len(p)]
+        if not parts:
+            return RawHits.empty()
+        fused = all(p.mm_mask is not None for p in parts)
+        return RawHits(np.concatenate([p.strand for p in parts]),
+                       np.concatenate([p.pos for p in parts]),
+                       np.concatenate([p.n_mm for p in parts]),
+                       np.concatenate([p.mm_mask for p in parts]) if fused else None,
+                       np.concatenate([p.score for p in parts]) if fused else None)
+
+
+def _parse_device(device) -> Tuple[str, int]:
+    if device is None:
+        return "cpu", 0
+    if hasattr(device, "type"):
+        return str(device.type), int(device.index or 0)
+    s = str(device)
+    kind, _, idx = s.partition(":")
+    return kind, int(idx) if idx else 0
+
+
+def _guide_planes_2bit(seq: str) -> int:
+    return encode_guide_64bit(encode_dna(seq)) & _W40
+
+
+class beguidedoff:
+    """Extensive (exhaustive GPU) off-target search."""
+
+    def __init__(self, genome_path: Optional[str] = None, max_mismatches: int = 4,
+                 scoring_method: str = "CCTOP", device: Optional[str] = None,
+                 n_top_offtargets: int = 3, results_per_guide: int = 10240,
+                 max_results_per_guide: Optional[int] = None, engine: str = "batched",
+                 genome: Optional[dict] = None, tuning: Optional[dict] = None):
+        self.genome_path = genome_path
+        self.max_mismatches = max_mismatches
+        if scoring_method.upper() != "CCTOP":
+            raise ValueError("extensive mode only supports scoring_method='CCTOP'")
+        self.scoring_method = "CCTOP"
+        self.n_top_offtargets = n_top_offtargets
+        self.results_per_guide = results_per_guide
+        self.max_results_per_guide = max_results_per_guide
+        if engine not in ("batched", "plain"):
+            raise ValueError(f"engine must be 'batched' or 'plain', got {engine!r}")
+        self.engine = engine
+        self.tuning = dict(tuning or {})
+
+        gpu_ok = True
+        try:
+            n_dev = _lib.device_count()
+        except _lib.LibraryUnavailable:
+            gpu_ok, n_dev = False, 0
+        if device is None:
+            device = "cuda:0" if n_dev > 0 else "cpu"
+        kind, index = _parse_device(device)
+        self.device = f"{kind}:{index}" if kind in ("cuda", "hip") else kind
+        if kind not in ("cuda", "hip") or not gpu_ok or index >= n_dev:
+            raise RuntimeError(
+                f"Extensive mode requires an MI355X GPU (device '{self.device}', "
+                f"{n_dev} HIP device(s), kernel library "
+                f"{'available' if gpu_ok else 'unavailable'}). Use the fast CPU mode "
+                "instead: `beguidedoff(mode='fast', index_path=...)`.")
+        self.device_index = index
+        _lib.init(index)
+
+        if genome is None:
+            if genome_path is None:
+                raise ValueError("genome_path (or a preloaded genome dict) is required")
+            genome = load_genome(genome_path)
+        self.genome_data = genome
+        self.total_bp = int(genome["total_bp"])
+        self.chrom_names = [str(c) for c in genome["chrom_names"]]
+        self.chrom_offsets = np.asarray(genome["chrom_offsets"], dtype=np.int64)
+        self.chrom_lengths = np.asarray(genome["chrom_lengths"], dtype=np.int64)
+        self._chrom_names_arr = np.array(self.chrom_names, dtype=object)
+        self._prepare_n_masks()
+        self._genome_handle = None   # batched engine: device genome, built lazily
+
+    @classmethod
+    def host_view(cls, genome: dict, max_mismatches: int = 4, n_top_offtargets: int = 3,
+                  engine: str = "batched") -> "beguidedoff":
+        """A finder without a device, for host-side decoding of hits produced
+        elsewhere (e.g. replaying recorded raw hits); it cannot scan."""
+        self = cls.__new__(cls)
+        self.genome_path = None
+        self.max_mismatches = max_mismatches
+        self.scoring_method = "CCTOP"
+        self.n_top_offtargets = n_top_offtargets
+        self.results_per_guide = _DEFAULT_RESULTS_PER_GUIDE
+        self.max_results_per_guide = None
+        self.engine = engine
+        self.tuning = {}
+        self.device = "none"
+        self.device_index = -1
+        self.genome_data = genome
+        self.total_bp = int(genome["total_bp"])
+        self.chrom_names = [str(c) for c in genome["chrom_names"]]
+        self.chrom_offsets = np.asarray(genome["chrom_offsets"], dtype=np.int64)
+        self.chrom_lengths = np.asarray(genome["chrom_lengths"], dtype=np.int64)
+        self._chrom_names_arr = np.array(self.chrom_names, dtype=object)
+        self._prepare_n_masks()
+        self._genome_handle = None
+        return self
+
+    def assemble(self, guides: List[GuideInput], raw_by_key: Dict[tuple, RawHits],
+                 return_all: bool = False,
+                 count_score_threshold: Optional[float] = None) -> pd.DataFrame:
+        """Table of `guides` from raw hits given per (guide_seq, pam) key."""
+        scan_guides, mapping = self._deduplicate_guides(guides)
+        raw = {sg: raw_by_key.get((sg.guide_seq, sg.pam), RawHits.empty()) for sg in scan_guides}
+        return assemble_results(self, guides, scan_guides, mapping, raw, count_score_threshold,
+                                return_all)
+
+    # ---------------------------------------------------------------- setup
+
+    def _prepare_n_masks(self) -> None:
+        """Little-endian N masks of both strand arrays, or None when the
+        artefact has no N (beguidedoff/finder.py:310-345)."""
+        g = self.genome_data
+        if "n_mask_le_fwd" in g and "n_mask_le_rc" in g:
+            fwd, rc = g["n_mask_le_fwd"], g["n_mask_le_rc"]
+            has_n = fwd is not None and bool(np.any(fwd))
+            self._n_mask_fwd = fwd if has_n else None
+            self._n_mask_rc = rc if has_n else None
+            return
+        self._n_mask_fwd, self._n_mask_rc = nmask_le(g)
+
+    @property
+    def genome_info(self) -> dict:
+        return {"build": "unknown", "total_bp": self.total_bp,
+                "n_chromosomes": len(self.chrom_names), "chromosomes": self.chrom_names}
+
+    def _strand_arrays(self):
+        out = [(0, "+", self.genome_data["genome_forward"], self._n_mask_fwd)]
+        if "genome_rc" in self.genome_data:
+            out.append((1, "-", self.genome_data["genome_rc"], self._n_mask_rc))
+        return out
+
+    def _deduplicate_guides(self, guides: List[GuideInput]):
+        """Scan keys and key -> inputs mapping (beguidedoff/finder.py:361-403)."""
+        mapping: Dict[_ScanGuide, List[GuideInput]] = defaultdict(list)
+        for g in guides:
+            ot_idx = ot_local = ot_strand = None
+            if g.on_target_chrom and g.on_target_pos is not None:
+                ot_idx = next((i for i, n in enumerate(self.chrom_names)
+                               if n == g.on_target_chrom), None)
+                if ot_idx is not None:
+                    if g.on_target_strand is None:
+                        raise ValueError(
+                            f"Guide {g.guide_id} has on-target chromosome and position but no "
+                            "strand. Please provide on_target_strand ('+' or '-').")
+                    window = 20 + len(g.pam)
+                    ot_local = (g.on_target_pos - 1 if g.on_target_strand == "+"
+                                else g.on_target_pos - window)
+                    ot_strand = g.on_target_strand
+            mapping[_ScanGuide(g.guide_seq, g.pam, ot_idx, ot_local, ot_strand)].append(g)
+        return list(mapping.keys()), mapping
+
+    # ---------------------------------------------------------------- search
+
+    def search(self, input_file: Optional[str] = None, guides: Optional[List[GuideInput]] = None,
+               return_all: bool = False, count_score_threshold: Optional[float] = None,
+               output_file: Optional[str] = None, include_non_editable: bool = False,
+               presets_path: Optional[str] = None, verbose: bool = True,
+               logger: Optional[logging.Logger] = None) -> pd.DataFrame:
+        """Search the genome for every guide (beguidedoff/finder.py:474-566).
+
+        ``include_non_editable`` is accepted and ignored, like the reference:
+        non-editable sites are kept exactly when ``return_all`` is set.
+        """
+        if (input_file is None) == (guides is None):
+            raise ValueError("Exactly one of input_file or guides must be provided")
+        if input_file is not None:
+            guides = load_guides_from_tsv(
+                input_file, presets_path=presets_path or str(_PROFILES_TSV))
+        if not guides:
+            raise ValueError("No guides provided")
+        log = logger if logger is not None else _LOGGER
+        for g in guides:
+            if g.pam_position.lower() != "3prime":
+                raise ValueError(f"Guide {g.guide_id} has pam_position '{g.pam_position}'. "
+                                 "Only 3prime PAM is supported.")
+        scan_guides, mapping = self._deduplicate_guides(guides)
+        if verbose:
+            log.info("Guides after deduplication: %d (from %d input rows)", len(scan_guides),
+                     len(guides))
+            log.info("Running GPU search on %s (%s engine) ...", self.device, self.engine)
+        if self.engine == "plain":
+            raw, overflow, _counts = self._search_gpu(scan_guides, log, verbose)
+        else:
+            raw, overflow, _counts = self._search_batched(scan_guides, log, verbose)
+        truncated = [sg for sg, o in overflow.items() if o]
+        if truncated:
+            log.warning("%d guide(s) returned truncated off-target lists (the re-scan "
+                        "escalation ceiling was reached); their off-target counts are lower "
+                        "bounds. See the per-guide warnings above.", len(truncated))
+        df = assemble_results(self, guides, scan_guides, mapping, raw, count_score_threshold,
+                              return_all)
+        if output_file:
+            df.to_csv(output_file, sep="\t", index=False)
+            if verbose:
+                log.info("Wrote results to %s", output_file)
+        return df
+
+    # ----------------------------------------------------- plain engine
+
+    def _search_strand(self, strand_idx: int, words, n_mask, scan_guides: List[_ScanGuide],
+                       results_per_guide: int):
+        """One strand, keys in launches of <= MAX_GUIDES (finder.py:119-249)."""
+        from .cuda_search import search_genome_cuda
+
+        if results_per_guide <= _DEFAULT_RESULTS_PER_GUIDE:
+            batch_limit = _MAX_GUIDES_PER_LAUNCH
+        else:
+            batch_limit = max(1, _DECODE_DICT_BUDGET // results_per_guide)
+        raw, ovf, counts = {}, {}, {}
+        for b0 in range(0, len(scan_guides), batch_limit):
+            batch = scan_guides[b0:b0 + batch_limit]
+            n = len(batch)
+            vals = np.zeros(n, dtype=np.uint64)
+            masks = np.zeros((n, 6), dtype=np.uint8)
+            lens = np.zeros(n, dtype=np.uint8)
+            for i, sg in enumerate(batch):
+                vals[i] = encode_guide_64bit(encode_dna(sg.guide_seq)) & ((1 << 64) - 1)
+                m = encode_pam_mask(sg.pam)
+                masks[i, :] = m
+                lens[i] = len(sg.pam)
+            res, o, c = search_genome_cuda(
+                words, self.total_bp, vals, masks, lens, np.ones(n, dtype=np.uint8),
+                self.max_mismatches, return_overflow=True, results_per_guide=results_per_guide,
+                n_mask_packed=n_mask, n_encoded=0 if strand_idx == 0 else 3)
+            for gi, sg in enumerate(batch):
+                hits = res[gi]
+                pos = np.fromiter((h[0] for h in hits), dtype=np.int64, count=len(hits))
+                mm = np.fromiter((h[1] for h in hits), dtype=np.int64, count=len(hits))
+                raw[sg] = RawHits(np.full(pos.size, strand_idx), pos, mm)
+                ovf[sg] = bool(o.get(gi, False))
+                counts[sg] = int(c.get(gi, 0))
+        return raw, ovf, counts
+
+    def _scan_strands(self, scan_guides, results_per_guide):
+        raw = {sg: [] for sg in scan_guides}
+        ovf = {sg: False for sg in scan_guides}
+        counts = {sg: 0 for sg in scan_guides}
+        for s, _label, words, n_mask in self._strand_arrays():
+            r, o, c = self._search_strand(s, words, n_mask, scan_guides, results_per_guide)
+            for sg in scan_guides:
+                raw[sg].append(r[sg])
+                ovf[sg] = ovf[sg] or o[sg]
+                counts[sg] += c[sg]
+        return {sg: RawHits.concat(v) for sg, v in raw.items()}, ovf, counts
+
+    def _search_gpu(self, scan_guides, log, verbose):
+        """Overflow escalation loop of beguidedoff/finder.py:568-638."""
+        raw = {sg: RawHits.empty() for sg in scan_guides}
+        overflow = {sg: False for sg in scan_guides}
+        counts = {sg: 0 for sg in scan_guides}
+        pending = list(scan_guides)
+        rpg = self.results_per_guide
+        while pending:
+            r, o, c = self._scan_strands(pending, rpg)
+            for sg in pending:
+                raw[sg], overflow[sg], counts[sg] = r[sg], o[sg], c[sg]
+            pending = [sg for sg in pending if o[sg]]
+            if not pending:
+                break
+            cap = (self.max_results_per_guide if self.max_results_per_guide is not None
+                   else _MAX_RESULTS_PER_GUIDE)
+            if rpg >= cap:
+                for sg in sorted(pending, key=lambda s: s.guide_seq):
+                    _warn_truncated(log, sg, counts[sg], rpg, len(raw[sg]))
+                break
+            rpg = min(rpg * 16, cap)
+            if log is not None:
+                names = ", ".join(sorted({sg.guide_seq for sg in pending})[:5])
+                log.warning("Hit buffer overflow for %d guide(s) at results_per_guide=%s "
+                            "(e.g. %s); re-scanning them with results_per_guide=%d.",
+                            len(pending), self.results_per_guide, names, rpg)
+        return raw, overflow, counts
+
+    # --------------------------------------------------- batched engine
+
+    def _device_genome(self):
+        if self._genome_handle is None:
+            from .batched import DeviceGenome
+
+            self._genome_handle = DeviceGenome(self.genome_data, self._n_mask_fwd,
+                                               self._n_mask_rc)
+        return self._genome_handle
+
+    def _search_batched(self, scan_guides, log, verbose):
+        """All keys, both strands, one persistent launch per PAM-family group.
+
+        Results are exact; a key whose sites on one strand exceed the
+        escalation ceiling keeps the lowest-position `ceiling` of them and is
+        reported as truncated (the reference keeps the first `ceiling` the
+        kernel stored, finder.py:609-625)."""
+        from .batched import GuideSpec, scan
+
+        specs = [GuideSpec(sg.guide_seq, sg.pam, self.max_mismatches) for sg in scan_guides]
+        res = scan(self._device_genome(), specs, tuning=self.tuning)
+        cap = (self.max_results_per_guide if self.max_results_per_guide is not None
+               else _MAX_RESULTS_PER_GUIDE)
+        hits = res.hits
+        order = np.lexsort((hits["pos"], hits["strand"], hits["guide"]))
+        hits = hits[order]
+        bounds = np.searchsorted(hits["guide"], np.arange(len(scan_guides) + 1), side="left")
+        raw, overflow, counts = {}, {}, {}
+        for k, sg in enumerate(scan_guides):
+            h = hits[bounds[k]:bounds[k + 1]]
+            counts[sg] = int(h.size)
+            keep = np.ones(h.size, dtype=bool)
+            over = False
+            for s in (0, 1):
+                idx = np.flatnonzero(h["strand"] == s)
+                if idx.size > cap:
+                    keep[idx[cap:]] = False
+                    over = True
+            if over:
+                h = h[keep]
+            raw[sg] = RawHits(h["strand"], h["pos"], h["n_mm"], h["mm_mask"], h["score"])
+            overflow[sg] = over
+            if over:
+                _warn_truncated(log, sg, counts[sg], cap, int(h.size))
+        return raw, overflow, counts
+
+    def close(self) -> None:
+        if self._genome_handle is not None:
+            self._genome_handle.close()
+            self._genome_handle = None
+
+
+def _warn_truncated(log, sg, count, rpg, kept):
+    log.warning("Truncated off-target enumeration for guide %s (PAM=%s): %d raw genome-wide "
+                "hits exceed the per-strand buffer of %d; keeping the first %d (genome order). "
+                "The off-target count for this guide is a lower bound.",
+                sg.guide_seq, sg.pam, count, rpg, kept)
+
+
+# ------------------------------------------------------------ epilogue
+
+
+def _codes_at(words: np.ndarray, pos: np.ndarray, width: int) -> np.ndarray:
+    """2-bit codes of bases [pos, pos + width) for every pos: (n, width) uint8."""
+    if pos.size == 0:
+        return np.zeros((0, width), dtype=np.uint8)
+    idx = pos[:, None] + np.arange(width, dtype=np.int64)[None, :]
+    w = np.asarray(words)[idx >> 4].astype(np.uint32)
+    return ((w >> ((idx & 15) * 2).astype(np.uint32)) & 3).astype(np.uint8)
+
+
+_ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+def _strings(codes: np.ndarray) -> np.ndarray:
+    """(n, w) codes -> object array of strings."""
+    n, w = codes.shape
+    if n == 0:
+        return np.zeros(0, dtype=object)
+    b = _ACGT[codes].reshape(n * w).tobytes()
+    return np.array([b[i * w:(i + 1) * w].decode("ascii") for i in range(n)], dtype=object)
+
+
+_MM_STR_CACHE: Dict[int, str] = {}
+
+
+def _mm_string(mask: int) -> str:
+    s = _MM_STR_CACHE.get(mask)
+    if s is None:
+        s = ",".join(str(i + 1) for i in range(20) if (mask >> i) & 1)
+        _MM_STR_CACHE[mask] = s
+    return s
+
+
+def cctop_scores(mm_mask: np.ndarray) -> np.ndarray:
+    """sum(1.2 ** p) over set positions, accumulated in ascending p: the same
+    IEEE additions as calculate_cctop_score (x + 0.0 leaves x unchanged)."""
+    tab = cctop_table()
+    s = np.zeros(mm_mask.size, dtype=np.float64)
+    for p in range(1, 21):
+        s = s + np.where((mm_mask >> np.uint32(p - 1)) & 1, tab[p], 0.0)
+    return s
+
+
+def _position_to_chrom(offsets: np.ndarray, pos: np.ndarray):
+    """Vector form of finder.py:36-40; index -1 wraps like Python indexing."""
+    idx = np.searchsorted(offsets, pos, side="right") - 1
+    return idx, pos - offsets[idx]
+
+
+@dataclass
+class _KeyHits:
+    n: int
+    strand: np.ndarray
+    report: np.ndarray      # 0-based global forward report position
+    seq_codes: np.ndarray   # (n, 20)
+    pam_str: np.ndarray     # object
+    n_mm: np.ndarray
+    mm_mask: np.ndarray
+    score: np.ndarray
+
+
+def decode_key_hits(finder: "beguidedoff", sg: _ScanGuide, raw: RawHits) -> _KeyHits:
+    """Coordinates, on-target exclusion, sequences, mismatch mask and score of
+    one key's raw hits (beguidedoff/finder.py:193-244, 424-439)."""
+    offs, lens = finder.chrom_offsets, finder.chrom_lengths
+    plen = len(sg.pam)
+    window = 20 + plen
+    strand = raw.strand
+    pos = raw.pos
+    leftmost = pos.copy()
+    report = pos.copy()
+    minus = strand == 1
+    if minus.any():
+        k, local_rc = _position_to_chrom(offs, pos[minus])
+        lm = offs[k] + (lens[k] - local_rc - window)
+        leftmost[minus] = lm
+        report[minus] = lm + window - 1
+    keep = np.ones(pos.size, dtype=bool)
+    if sg.on_target_chrom_idx is not None:
+        k, local = _position_to_chrom(offs, leftmost)
+        s_code = 0 if sg.on_target_strand == "+" else (1 if sg.on_target_strand == "-" else -9)
+        keep &= ~((k == sg.on_target_chrom_idx) & (local == sg.on_target_local_pos)
+                  & (strand == s_code))
+    strand, pos, report = strand[keep], pos[keep], report[keep]
+    n_mm = raw.n_mm[keep]
+    seq = np.zeros((pos.size, 20), dtype=np.uint8)
+    pam = np.zeros((pos.size, plen), dtype=np.uint8)
+    for s, _label, words, _nm in finder._strand_arrays():
+        sel = strand == s
+        if sel.any():
+            seq[sel] = _codes_at(words, pos[sel], 20)
+            pam[sel] = _codes_at(words, pos[sel] + 20, plen)
+    g = encode_dna(sg.guide_seq)[:20]
+    g = np.pad(g, (0, 20 - g.size))
+    diff = (seq != g[None, :])
+    mm_mask = (diff.astype(np.uint32) << np.arange(20, dtype=np.uint32)[None, :]).sum(
+        axis=1, dtype=np.uint64).astype(np.uint32)
+    if raw.score is not None:
+        score = raw.score[keep]
+        if not np.array_equal(raw.mm_mask[keep], mm_mask):
+            raise RuntimeError("fused epilogue mismatch mask disagrees with the genome")
+    else:
+        score = cctop_scores(mm_mask)
+    return _KeyHits(int(pos.size), strand, report, seq, _strings(pam), n_mm, mm_mask, score)
+
+
+def assemble_results(finder: "beguidedoff", guides: List[GuideInput],
+                     scan_guides: List[_ScanGuide], mapping, raw: Dict[_ScanGuide, RawHits],
+                     count_score_threshold: Optional[float], return_all: bool) -> pd.DataFrame:
+    """Per-input-guide sort, top-N and table (beguidedoff/finder.py:405-472, 545-566).
+
+    Hits are collected per guide_id (inputs sharing an id are merged, as the
+    reference's id-keyed dicts merge them) and ordered by (score, chromosome
+    name, position, strand); remaining ties keep the reference's insertion
+    order (scan key, then input order).
+    """
+    offs = finder.chrom_offsets
+    names = finder._chrom_names_arr
+    uniq = sorted(set(finder.chrom_names))
+    rank_of = {n: i for i, n in enumerate(uniq)}
+    chrom_rank = np.array([rank_of[n] for n in finder.chrom_names], dtype=np.int64)
+    key_index = {sg: i for i, sg in enumerate(scan_guides)}
+    decoded = {sg: decode_key_hits(finder, sg, raw.get(sg, RawHits.empty())) for sg in scan_guides}
+    input_index = {id(g): i for i, g in enumerate(guides)}
+
+    per_id: Dict[str, list] = defaultdict(list)
+    below: Dict[str, int] = defaultdict(int)
+    for sg in scan_guides:
+        kh = decoded[sg]
+        if kh.n == 0:
+            continue
+        for g in mapping[sg]:
+            target = EDITABLE_BASE.get(g.editor_type)
+            if target is None:
+                raise ValueError(f"Unknown editor_type: {g.editor_type}")
+            wmask = edit_window_mask(g.edit_window_start, g.edit_window_end)
+            base_is = kh.seq_codes == {"A": 0, "C": 1}[target]
+            bits = (base_is.astype(np.uint32) << np.arange(20, dtype=np.uint32)[None, :]).sum(
+                axis=1, dtype=np.uint64).astype(np.uint32)
+            editable = (bits & np.uint32(wmask)) != 0
+            if count_score_threshold is not None:
+                below[g.guide_id] += int(np.count_nonzero(
+                    editable & (kh.score <= count_score_threshold)))
+            idx = np.flatnonzero(editable | return_all)
+            if idx.size:
+                per_id[g.guide_id].append((key_index[sg], input_index[id(g)], sg, g, idx))
+
+    frames = []
+    cache: Dict[str, Optional[pd.DataFrame]] = {}
+    for g in guides:
+        gid = g.guide_id
+        if gid not in cache:
+            cache[gid] = _id_frame(finder, decoded, per_id.get(gid, []), below.get(gid, 0),
+                                   return_all, chrom_rank, names, offs)
+        if cache[gid] is not None:
+            frames.append(cache[gid])
+    if not frames:
+        return pd.DataFrame()
+    return pd.concat(frames, ignore_index=True)
+
+
+def _id_frame(finder, decoded, parts, n_below, return_all, chrom_rank, names, offs):
+    if not parts:
+        return None
+    cols = defaultdict(list)
+    for kidx, iidx, sg, g, idx in parts:
+        kh = decoded[sg]
+        k, local = _position_to_chrom(offs, kh.report[idx])
+        cols["score"].append(kh.score[idx])
+        cols["crank"].append(chrom_rank[k])
+        cols["chrom"].append(k)
+        cols["local"].append(local)
+        cols["strand"].append(kh.strand[idx])
+        cols["key"].append(np.full(idx.size, kidx))
+        cols["inp"].append(np.full(idx.size, iidx))
+        cols["seq"].append(kh.seq_codes[idx])
+        cols["pam"].append(kh.pam_str[idx])
+        cols["n_mm"].append(kh.n_mm[idx])
+        cols["mm"].append(kh.mm_mask[idx])
+        cols["gseq"].append(np.full(idx.size, sg.guide_seq, dtype=object))
+        cols["editor"].append(np.full(idx.size, g.base_editor or "", dtype=object))
+    c = {k: np.concatenate(v) for k, v in cols.items()}
+    order = np.lexsort((c["inp"], c["key"], c["strand"], c["local"], c["crank"], c["score"]))
+    if not return_all:
+        order = order[: finder.n_top_offtargets]
+    if order.size == 0:
+        return None
+    return pd.DataFrame({
+        "guide_rna_sequence": c["gseq"][order],
+        "base_editor": c["editor"][order],
+        "ot_rank": np.arange(1, order.size + 1, dtype=np.int64),
+        "ot_chromosome": names[c["chrom"][order]],
+        "ot_position": (c["local"][order] + 1).astype(np.int64),
+        "ot_strand": np.where(c["strand"][order] == 0, "+", "-").astype(object),
+        "ot_sequence": _strings(c["seq"][order]),
+        "ot_pam": c["pam"][order],
+        "n_mismatches": c["n_mm"][order].astype(np.int64),
+        "mismatch_positions": np.array([_mm_string(int(m)) for m in c["mm"][order]],
+                                       dtype=object),
+        "score": np.array([round(float(x), 6) for x in c["score"][order]], dtype=np.float64),
+        "n_offtargets_below_threshold": np.full(order.size, n_below, dtype=np.int64),
+    }, columns=OUTPUT_COLUMNS)

@@ -1,0 +1,345 @@
+"""Synthetic workloads for the extensive scan.
+
+Genomes are i.i.d. bases at 41% GC (P(A)=P(T)=0.295, P(C)=P(G)=0.205) from a
+counter-based hash of (seed, position) computed natively (libbgo_host.so),
+with N runs and optional planted guide sites.  The result carries the
+reference's artefact keys (``genome_forward``, ``genome_rc``, ``chrom_*``,
+``total_bp``, ``n_mask``) plus the little-endian N masks the scan consumes.
+
+Workload definitions (sizes, PAM/editor mixes) are those of the benchmark
+configurations this package is measured on; see :func:`workload`.
+"""
+from __future__ import annotations
+
+import bisect
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .pam import IUPAC_MASK
+from .types import GuideInput
+
+HOST_LIB = Path(__file__).resolve().parent / "libbgo_host.so"
+
+SMALL_CONTIGS = [("chr1", 5_000_000), ("chr2", 3_000_000), ("chr3", 2_000_000)]
+
+# GRCh38 primary chromosome lengths (sum 3,088,269,832 bp).
+HG38_LIKE_CONTIGS = [
+    ("chr1", 248_956_422), ("chr2", 242_193_529), ("chr3", 198_295_559),
+    ("chr4", 190_214_555), ("chr5", 181_538_259), ("chr6", 170_805_979),
+    ("chr7", 159_345_973), ("chr8", 145_138_636), ("chr9", 138_394_717),
+    ("chr10", 133_797_422), ("chr11", 135_086_622), ("chr12", 133_275_309),
+    ("chr13", 114_364_328), ("chr14", 107_043_718), ("chr15", 101_991_189),
+    ("chr16", 90_338_345), ("chr17", 83_257_441), ("chr18", 80_373_285),
+    ("chr19", 58_617_616), ("chr20", 64_444_167), ("chr21", 46_709_983),
+    ("chr22", 50_818_468), ("chrX", 156_040_895), ("chrY", 57_227_415),
+]
+
+_host = None
+
+
+def host_lib() -> C.CDLL:
+    global _host
+    if _host is None:
+        if not HOST_LIB.exists():
+            raise RuntimeError(f"{HOST_LIB.name} is not built; run `python -m beguidedoff_amd.build`")
+        lib = C.CDLL(str(HOST_LIB))
+        P = C.c_void_p
+        lib.bgo_synth_fill.restype = C.c_int
+        lib.bgo_synth_fill.argtypes = [C.c_uint64, C.c_int64, P, P, C.c_int32, P, P, C.c_int64,
+                                       P, P, P, C.c_int64, P, P, P, P, P]
+        lib.bgo_synth_extract.restype = None
+        lib.bgo_synth_extract.argtypes = [P, C.c_int64, C.c_int64, P]
+        lib.bgo_nmask_convert.restype = C.c_int64
+        lib.bgo_nmask_convert.argtypes = [P, C.c_int64, P, P, C.c_int32, P, P]
+        _host = lib
+    return _host
+
+
+def _p(a: Optional[np.ndarray]) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else C.c_void_p(0)
+
+
+# ------------------------------------------------------------------ layout
+
+
+def contig_layout(contigs: Sequence[Tuple[str, int]]):
+    names = [n for n, _ in contigs]
+    lengths = np.array([l for _, l in contigs], dtype=np.int64)
+    offsets = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.int64)
+    return names, offsets, lengths
+
+
+def place_n_runs(rng: np.random.Generator, offsets: np.ndarray, lengths: np.ndarray,
+                 frac: float, min_len: int, max_len: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Non-overlapping N runs covering about `frac` of every contig."""
+    starts, ends = [], []
+    for off, clen in zip(offsets.tolist(), lengths.tolist()):
+        budget = int(round(frac * clen))
+        blocks = []
+        while sum(blocks) < budget:
+            blocks.append(int(min(rng.integers(min_len, max_len + 1), budget - sum(blocks))))
+        if not blocks:
+            continue
+        free = clen - sum(blocks)
+        if free <= 0:
+            continue
+        cuts = np.sort(rng.integers(0, free + 1, size=len(blocks)))
+        acc = 0
+        for cut, blen in zip(cuts.tolist(), blocks):
+            s = off + cut + acc
+            starts.append(s)
+            ends.append(s + blen)
+            acc += blen
+    order = np.argsort(starts, kind="stable")
+    return np.asarray(starts, np.int64)[order], np.asarray(ends, np.int64)[order]
+
+
+def random_guides(rng: np.random.Generator, n: int, length: int = 20) -> List[str]:
+    codes = rng.integers(0, 4, size=(n, length))
+    alphabet = np.array(list("ACGT"))
+    return ["".join(row) for row in alphabet[codes]]
+
+
+# ------------------------------------------------------------------ plants
+
+
+@dataclass
+class Plant:
+    guide: int         # index into the guide list
+    pos: int           # global forward leftmost position of the site + PAM
+    strand: str        # orientation of the guide site
+    n_mm: int          # mismatches planted in the 20-nt site
+    mm_positions: Tuple[int, ...]   # 1-based
+    codes: np.ndarray  # forward-oriented 2-bit codes written at pos
+
+
+_CODE = {"A": 0, "C": 1, "G": 2, "T": 3}
+
+
+def _resolve_pam(rng: np.random.Generator, pam: str) -> List[int]:
+    out = []
+    for ch in pam.upper():
+        allowed = [b for b in range(4) if IUPAC_MASK[ch] & (1 << b)]
+        out.append(int(allowed[rng.integers(0, len(allowed))]))
+    return out
+
+
+def plan_plants(rng: np.random.Generator, offsets: np.ndarray, lengths: np.ndarray,
+                n_starts: np.ndarray, n_ends: np.ndarray, guides: Sequence[Tuple[str, str]],
+                copies: int, max_mm: int) -> List[Plant]:
+    """`copies` near-copies of every guide: k ~ U{0..max_mm} substitutions at
+    distinct uniformly chosen positions of the 20-nt site, a PAM resolved to a
+    random allowed base per IUPAC code, a random strand; N-free,
+    contig-internal and non-overlapping."""
+    total = int(lengths.sum())
+    occ_s: List[int] = []
+    occ_e: List[int] = []
+    plants: List[Plant] = []
+    cum = np.cumsum(lengths)
+    for gi, (seq, pam) in enumerate(guides):
+        g = [_CODE.get(c, 0) for c in seq.upper()[:20]]
+        g += [0] * (20 - len(g))
+        win = 20 + len(pam)
+        for _ in range(copies):
+            for _attempt in range(1000):
+                x = int(rng.integers(0, total))
+                k_idx = int(np.searchsorted(cum, x, side="right"))
+                off, clen = int(offsets[k_idx]), int(lengths[k_idx])
+                if clen < win + 2:
+                    continue
+                pos = off + int(rng.integers(0, clen - win))
+                j = int(np.searchsorted(n_ends, pos, side="right"))
+                if j < len(n_starts) and n_starts[j] < pos + win:
+                    continue
+                b = bisect.bisect_left(occ_s, pos)
+                if b > 0 and occ_e[b - 1] > pos:
+                    continue
+                if b < len(occ_s) and occ_s[b] < pos + win:
+                    continue
+                break
+            else:
+                raise RuntimeError("could not place a planted site")
+            k = int(rng.integers(0, max_mm + 1))
+            mm = sorted(int(i) for i in rng.choice(20, size=k, replace=False)) if k else []
+            site = list(g)
+            for i in mm:
+                site[i] = int((site[i] + rng.integers(1, 4)) % 4)
+            oriented = np.array(site + _resolve_pam(rng, pam), dtype=np.uint8)
+            strand = "+" if rng.integers(0, 2) == 0 else "-"
+            fwd = oriented if strand == "+" else (3 - oriented[::-1]).astype(np.uint8)
+            bisect.insort(occ_s, pos)
+            occ_e.insert(bisect.bisect_left(occ_s, pos), pos + win)
+            plants.append(Plant(gi, pos, strand, k, tuple(i + 1 for i in mm), fwd))
+    return plants
+
+
+# ------------------------------------------------------------------ genome
+
+
+def make_genome(contigs: Sequence[Tuple[str, int]], seed: int = 0, n_frac: float = 0.0,
+                n_len: Tuple[int, int] = (1000, 10000), plants: Optional[List[Plant]] = None,
+                n_runs: Optional[Tuple[np.ndarray, np.ndarray]] = None,
+                with_rc: bool = True) -> Dict[str, np.ndarray]:
+    names, offsets, lengths = contig_layout(contigs)
+    total = int(lengths.sum())
+    if n_runs is None:
+        rng = np.random.default_rng([seed, 1])
+        n_runs = place_n_runs(rng, offsets, lengths, n_frac, *n_len) if n_frac > 0 else (
+            np.zeros(0, np.int64), np.zeros(0, np.int64))
+    n_s, n_e = (np.ascontiguousarray(a, dtype=np.int64) for a in n_runs)
+    n_words = (total + 15) // 16
+    n_bytes = (total + 7) // 8
+    fwd = np.empty(n_words, dtype=np.uint32)
+    rc = np.empty(n_words, dtype=np.uint32) if with_rc else None
+    nm_f = np.empty(n_bytes + 16, dtype=np.uint8)
+    nm_r = np.empty(n_bytes + 16, dtype=np.uint8)
+    nm_b = np.empty(n_bytes, dtype=np.uint8)
+    plants = plants or []
+    p_pos = np.array([p.pos for p in plants], dtype=np.int64)
+    p_len = np.array([len(p.codes) for p in plants], dtype=np.int32)
+    p_codes = (np.concatenate([p.codes for p in plants]).astype(np.uint8) if plants
+               else np.zeros(0, np.uint8))
+    rc_arg = _p(rc) if rc is not None else C.c_void_p(0)
+    r = host_lib().bgo_synth_fill(
+        C.c_uint64(seed & (2**64 - 1)), total, _p(offsets), _p(lengths), len(names), _p(n_s),
+        _p(n_e), n_s.size, _p(p_pos), _p(p_codes), _p(p_len), len(plants), _p(fwd), rc_arg,
+        _p(nm_f), _p(nm_r), _p(nm_b))
+    if r != 0:
+        raise RuntimeError("bgo_synth_fill failed")
+    nm_f[n_bytes:] = 0
+    nm_r[n_bytes:] = 0
+    out = {
+        "genome_forward": fwd,
+        "chrom_names": np.array(names),
+        "chrom_offsets": offsets,
+        "chrom_lengths": lengths,
+        "total_bp": total,
+        "n_mask": nm_b,
+        "n_mask_le_fwd": nm_f,
+        "n_mask_le_rc": nm_r,
+        "n_runs": (n_s, n_e),
+        "plants": plants,
+    }
+    if rc is not None:
+        out["genome_rc"] = rc
+    return out
+
+
+def genome_from_strings(contigs: Sequence[Tuple[str, str]]) -> Dict[str, np.ndarray]:
+    """Artefact arrays of explicit contig strings (same keys as make_genome)."""
+    from .genome_converter import encode_contigs
+
+    g = encode_contigs(list(contigs))
+    g["total_bp"] = int(g["total_bp"])
+    le_f, le_r = nmask_le(g)
+    g["n_mask_le_fwd"], g["n_mask_le_rc"] = le_f, le_r
+    return g
+
+
+def nmask_le(genome: dict) -> Tuple[Optional[np.ndarray], Optional[np.ndarray]]:
+    """Little-endian forward / reverse-complement N masks (+16 zero bytes) of an
+    artefact's big-endian ``n_mask``; (None, None) when the genome has no N
+    (beguidedoff/finder.py:310-345)."""
+    big = genome.get("n_mask")
+    total = int(genome["total_bp"])
+    if big is None or np.asarray(big).size == 0 or total == 0:
+        return None, None
+    big = np.ascontiguousarray(big, dtype=np.uint8)
+    n_bytes = (total + 7) // 8
+    if big.size < n_bytes:
+        big = np.pad(big, (0, n_bytes - big.size))
+    offs = np.ascontiguousarray(genome["chrom_offsets"], dtype=np.int64)
+    lens = np.ascontiguousarray(genome["chrom_lengths"], dtype=np.int64)
+    fwd = np.zeros(n_bytes + 16, dtype=np.uint8)
+    rc = np.zeros(n_bytes + 16, dtype=np.uint8)
+    n = host_lib().bgo_nmask_convert(_p(big), total, _p(offs), _p(lens), offs.size, _p(fwd),
+                                     _p(rc))
+    if n == 0:
+        return None, None
+    return fwd, rc
+
+
+def extract_codes(words: np.ndarray, start: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint8)
+    host_lib().bgo_synth_extract(_p(np.ascontiguousarray(words, dtype=np.uint32)), start, n,
+                                 _p(out))
+    return out
+
+
+# --------------------------------------------------------------- workloads
+
+
+@dataclass
+class Workload:
+    name: str
+    genome: Dict[str, np.ndarray]
+    guides: List[GuideInput]
+    max_mismatches: int
+    description: str
+
+
+def _guide_inputs(seqs: Sequence[str], pams: Sequence[str], windows: Sequence[Tuple[str, int, int]]
+                  ) -> List[GuideInput]:
+    return [GuideInput(f"g{i}", s, p, "3prime", w[1], w[2], w[0])
+            for i, (s, p, w) in enumerate(zip(seqs, pams, windows))]
+
+
+def mixed_guides(rng: np.random.Generator, n_ngg: int, n_nnnrrt: int, n_nngrrt: int,
+                 n_ng: int) -> List[GuideInput]:
+    """The mixed PAM / editor set: SpCas9-style 20-nt NGG, SaKKH-style 21-nt
+    NNNRRT, 21-nt NNGRRT and 20-nt NG; alternate guides ABE (window 4-8) and
+    CBE (window 3-9)."""
+    seqs, pams = [], []
+    for n, length, pam in ((n_ngg, 20, "NGG"), (n_nnnrrt, 21, "NNNRRT"),
+                           (n_nngrrt, 21, "NNGRRT"), (n_ng, 20, "NG")):
+        seqs += random_guides(rng, n, length)
+        pams += [pam] * n
+    windows = [("ABE", 4, 8) if i % 2 == 0 else ("CBE", 3, 9) for i in range(len(seqs))]
+    return _guide_inputs(seqs, pams, windows)
+
+
+def workload(name: str, seed: int = 0, scale: float = 1.0) -> Workload:
+    """Named workloads.
+
+    small      3 contigs (5, 3, 2 Mbp), 1% N in runs of 1-10 kb; 8 random 20-nt
+               NGG guides (rng seed 0), ABE 4-8, <= 3 mm.
+    hg38_ngg   24 hg38-like contigs (3.09 Gbp), ~5% N in runs of 10 kb-1 Mb;
+               128 random 20-nt NGG guides, ABE 4-8, <= 4 mm, 40 planted
+               near-copies (0-4 mm) per guide.
+    hg38_mixed 1024 guides (512 NGG/20 nt, 256 NNNRRT/21 nt, 128 NNGRRT/21 nt,
+               128 NG/20 nt), half ABE 4-8, half CBE 3-9, <= 5 mm.
+    hg38_8k    8192 random 20-nt NGG guides, ABE 4-8, <= 4 mm.
+    `scale` < 1 shrinks every contig (for quick runs and tests).
+    """
+    rng = np.random.default_rng(seed)
+    if name == "small":
+        contigs = [(n, max(64, int(l * scale))) for n, l in SMALL_CONTIGS]
+        seqs = random_guides(np.random.default_rng(0), 8)
+        guides = _guide_inputs(seqs, ["NGG"] * 8, [("ABE", 4, 8)] * 8)
+        genome = make_genome(contigs, seed=seed, n_frac=0.01, n_len=(1000, 10000))
+        return Workload(name, genome, guides, 3, "3 contigs 10 Mbp, 8 NGG guides, <=3 mm")
+    contigs = [(n, max(1000, int(l * scale))) for n, l in HG38_LIKE_CONTIGS]
+    names, offsets, lengths = contig_layout(contigs)
+    n_runs = place_n_runs(np.random.default_rng([seed, 1]), offsets, lengths, 0.05,
+                          max(10, int(10_000 * min(1.0, scale * 10))),
+                          max(20, int(1_000_000 * scale)))
+    if name == "hg38_ngg":
+        seqs = random_guides(rng, 128)
+        guides = _guide_inputs(seqs, ["NGG"] * 128, [("ABE", 4, 8)] * 128)
+        plants = plan_plants(rng, offsets, lengths, *n_runs, [(s, "NGG") for s in seqs], 40, 4)
+        genome = make_genome(contigs, seed=seed, n_runs=n_runs, plants=plants)
+        return Workload(name, genome, guides, 4, "hg38-like 3.09 Gbp, 128 NGG guides, <=4 mm")
+    if name == "hg38_mixed":
+        guides = mixed_guides(rng, 512, 256, 128, 128)
+        genome = make_genome(contigs, seed=seed, n_runs=n_runs)
+        return Workload(name, genome, guides, 5, "hg38-like 3.09 Gbp, 1024 mixed-PAM guides, <=5 mm")
+    if name == "hg38_8k":
+        seqs = random_guides(rng, 8192)
+        guides = _guide_inputs(seqs, ["NGG"] * 8192, [("ABE", 4, 8)] * 8192)
+        genome = make_genome(contigs, seed=seed, n_runs=n_runs)
+        return Workload(name, genome, guides, 4, "hg38-like 3.09 Gbp, 8192 NGG guides, <=4 mm")
+    raise ValueError(f"unknown workload {name!r}")

@@ -1,0 +1,219 @@
+/*
+ * bgo.h — C ABI of libbgo_hip.so, the MI355X (gfx950) extensive-mode
+ * off-target scan.
+ *
+ * Plain C types only (no torch, no HIP types).  Every entry point returns an
+ * int status: BGO_OK (0) on success, a negative BGO_ERR_* code on failure, in
+ * which case bgo_last_error() (thread-local) holds a message.  Python maps any
+ * non-zero status to RuntimeError, which is what the reference raises for an
+ * unusable GPU (beguidedoff/finder.py:154-155, 283-290) and for too many
+ * guides (beguidedoff/cuda_search.py:411-417).
+ *
+ * Coordinates: a "strand array" is the reference's 2-bit packed genome
+ * (16 nt per uint32, base i at bits 2(i%16)..+1 of word i/16, A0 C1 G2 T3,
+ * N encoded as A; beguidedoff/encoding.py:36-46) of either the forward genome
+ * or the per-contig reverse complement (genome_converter.py:66-71,92-95).
+ * N masks are little-endian bit masks, bit i = position i of that strand
+ * array (beguidedoff/finder.py:310-345).  Positions are 0-based offsets into
+ * the scanned strand array.
+ */
+#ifndef BGO_H
+#define BGO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BGO_OK 0
+#define BGO_ERR_ARG (-1)      /* invalid argument (sizes, pointers, limits) */
+#define BGO_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define BGO_ERR_NODEV (-3)    /* no usable gfx950 device */
+#define BGO_ERR_INJECTED (-4) /* failure forced by bgo_debug_fail_next() */
+
+/* Limits of the plain (drop-in) kernel; cuda_search.py:40-41. */
+#define BGO_PLAIN_MAX_GUIDES 2048
+#define BGO_MAX_PAM_LEN 6
+
+/* ---------------------------------------------------------------- library */
+
+/* ABI version of this header (bumped on any incompatible change). */
+int bgo_abi_version(void);
+
+/* Number of visible HIP devices (0 when none; never fails on a host without
+ * a GPU).  Replaces the torch.cuda.is_available() probe of
+ * beguidedoff/finder.py:279-290. */
+int bgo_device_count(int32_t* n_out);
+
+/* Select the device used by every later call of this thread and create the
+ * library's per-device state.  Idempotent. */
+int bgo_init(int32_t device);
+
+/* Message of the last failed call on this thread ("" when none). */
+const char* bgo_last_error(void);
+
+/* Test hook (no reference counterpart): the next `n_calls`-th entry point
+ * that would touch the GPU fails with BGO_ERR_INJECTED before doing any work.
+ * Lets CPU-only tests check the Python RuntimeError mapping. 0 disarms. */
+int bgo_debug_fail_next(int32_t n_calls);
+
+/* ------------------------------------------------------ one strand array */
+
+typedef struct bgo_strand bgo_strand;
+
+/* Upload one strand array and its optional N mask.  Replaces the identity-
+ * keyed upload caches _genome_to_device / _n_mask_to_device /
+ * _prepare_n_mask_device (beguidedoff/cuda_search.py:321-381): the caller
+ * owns the handle instead of a module-level cache.  nmask_le may be NULL (or
+ * nmask_bytes 0): then the scan has no N gate, like an empty mask in the
+ * reference.  The mask is zero-padded on the device to (total_bp >> 3) + 16
+ * bytes. */
+int bgo_strand_create(const uint32_t* words, int64_t n_words,
+                      const uint8_t* nmask_le, int64_t nmask_bytes,
+                      int64_t total_bp, bgo_strand** out);
+int bgo_strand_destroy(bgo_strand* s);
+
+/* Plain drop-in scan of one strand array == search_genome_cuda
+ * (beguidedoff/cuda_search.py:384-488) with the kernel of
+ * beguidedoff/cuda_search.py:163-314: one lane per position, the guide table
+ * in LDS, per guide a packed PAM test, a 40-bit XOR-fold-popcount and the
+ * N correction, a per-guide atomic append.  Launched per `chunk_size`
+ * positions over [0, total_bp - 26).
+ *   guides     uint64[n]    2-bit packed guide (first 20 nt are compared)
+ *   pam_masks  uint8[n][6]  IUPAC allowed-base nibbles (exotic-PAM fallback)
+ *   pats, cmps uint32[n]    encode_pam_packed() fields (cuda_search.py:124-160)
+ *   n_guides   <= BGO_PLAIN_MAX_GUIDES, else BGO_ERR_ARG
+ *   rpg        results recorded per guide (the count keeps going past it)
+ *   n_encoded  2-bit code an N reads as on this strand (0 on '+', 3 on '-')
+ * Outputs (host memory, caller-allocated): counts int32[n] exact raw match
+ * counts; positions int64[n][rpg] and mismatches int8[n][rpg], of which
+ * entries [0, min(count, rpg)) of each row are written. */
+int bgo_scan_plain(bgo_strand* s, const uint64_t* guides,
+                   const uint8_t* pam_masks, const uint32_t* pats,
+                   const uint32_t* cmps, int32_t n_guides,
+                   int32_t max_mismatches, int64_t chunk_size, int64_t rpg,
+                   int32_t n_encoded, int32_t* counts, int64_t* positions,
+                   int8_t* mismatches);
+
+/* Milliseconds the kernels of the last bgo_scan_plain on this thread took
+ * (HIP events on the scan stream). */
+int bgo_scan_plain_last_ms(float* ms_out);
+
+/* --------------------------------------------------- two-strand genome */
+
+typedef struct bgo_genome bgo_genome;
+
+/* Upload a genome for the batched engine: the forward array and (optional)
+ * the reverse-complement array with their N masks, all as above.  The
+ * library also derives the bit-plane layout the batched kernel reads (two
+ * uint32 planes per 32 bases: bit0 and bit1 of each base code).  Replaces
+ * the device copies of beguidedoff/finder.py:299-308 / cuda_search.py:325-362.
+ * total_bp must be below 2^32 - 64. */
+int bgo_genome_create(const uint32_t* words_fwd, const uint32_t* words_rc,
+                      int64_t n_words, const uint8_t* nmask_fwd,
+                      const uint8_t* nmask_rc, int64_t nmask_bytes,
+                      int64_t total_bp, bgo_genome** out);
+int bgo_genome_destroy(bgo_genome* g);
+/* Device bytes held by the genome (for admission control). */
+int bgo_genome_device_bytes(const bgo_genome* g, int64_t* bytes_out);
+
+/* One guide of a batched scan.  Guides of any PAM (IUPAC, 1-6 nt, 3' of the
+ * protospacer), editor and max-mismatch budget may be mixed in one launch;
+ * the library groups them by PAM family. */
+typedef struct bgo_guide_desc {
+  uint64_t seq2bit;      /* encode_guide_64bit(encode_dna(seq)); 20 nt used */
+  uint8_t pam_mask[6];   /* encode_pam_mask(pam) (pam.py:38-45) */
+  uint8_t pam_len;       /* 1..6 */
+  uint8_t max_mm;        /* mismatch budget incl. the N correction, <= 20 */
+  uint32_t edit_mask;    /* bit i set: window index i (0-based) is checked */
+  uint8_t editor;        /* 0: editable base 'A' (ABE); 1: 'C' (CBE/CGE) */
+  uint8_t pad[3];
+} bgo_guide_desc;        /* 24 bytes */
+
+/* One reported site.  Fields follow the extensive finder's per-hit values:
+ * pos and strand as the kernel reports them (cuda_search.py:310-314), n_mm
+ * with the N correction (cuda_search.py:290-308), mm_mask and score from the
+ * 2-bit window (finder.py:435-439, scoring.py:5-24), editable as
+ * editable_base_in_window() (base_editing.py:9-32) for the guide's editor. */
+typedef struct bgo_hit {
+  uint32_t guide;    /* index into the launch's guide array */
+  uint32_t pos;      /* 0-based position in the scanned strand array */
+  uint32_t x0;       /* bit0 planes of bases pos..pos+25 (window + PAM) */
+  uint32_t x1;       /* bit1 planes of the same bases */
+  uint32_t mm_mask;  /* bit i: window base i differs from guide base i */
+  uint8_t strand;    /* 0 '+', 1 '-' */
+  uint8_t n_mm;      /* N-corrected mismatch count */
+  uint8_t editable;  /* 1 when an editable base lies in the edit window */
+  uint8_t pad;
+  double score;      /* CCTOP: sum of table[i+1] over set mm_mask bits */
+} bgo_hit;           /* 32 bytes */
+
+/* Kernel selection and tiling of the batched engine (0 = library default /
+ * autotuned value). */
+typedef struct bgo_tuning {
+  int32_t chunk_positions;   /* positions per work item (multiple of 64) */
+  int32_t waves_per_block;   /* 1..16 */
+  int32_t blocks_per_cu;     /* resident blocks requested per CU */
+  int32_t engine;            /* 0 auto, 1 VALU xor/popcount, 2 int8 MFMA */
+  int32_t guide_unroll;      /* 4 or 8 (VALU engine) */
+  int32_t reserved[3];
+} bgo_tuning;
+
+typedef struct bgo_plan bgo_plan;
+
+/* Prepare a batched scan: sort/group the guides by PAM family on the host,
+ * upload the guide table, allocate a hit buffer of hit_capacity records and a
+ * private HIP stream.  score_table[0..20]: the per-position CCTOP penalty,
+ * score_table[p] = 1.2 ** p for p = 1..20 as computed by the host.
+ * tuning may be NULL. */
+int bgo_plan_create(bgo_genome* g, const bgo_guide_desc* guides,
+                    int32_t n_guides, const double* score_table,
+                    const bgo_tuning* tuning, int64_t hit_capacity,
+                    bgo_plan** out);
+int bgo_plan_destroy(bgo_plan* p);
+
+/* Enqueue one scan of positions [pos_begin, pos_end) (clamped to
+ * [0, total_bp - 26), cuda_search.py:211-213) on the strands of strand_mask
+ * (bit0 '+', bit1 '-') on the plan's stream.  Returns without waiting. */
+int bgo_plan_launch(bgo_plan* p, int64_t pos_begin, int64_t pos_end,
+                    int32_t strand_mask);
+
+/* Wait for the last launch; n_hits_out = exact number of sites found (may
+ * exceed the capacity: then only the first `capacity` were stored and the
+ * caller re-plans with a larger buffer); kernel_ms_out = kernel time
+ * measured with HIP events on the plan's stream (may be NULL). */
+int bgo_plan_wait(bgo_plan* p, int64_t* n_hits_out, float* kernel_ms_out);
+
+/* Copy min(n_hits, capacity, cap) records of the last launch to host memory. */
+int bgo_plan_fetch(bgo_plan* p, bgo_hit* out, int64_t cap, int64_t* n_copied);
+
+/* Capture [counter reset + scan of both strands over the full genome] in a
+ * hipGraph once, then replay it with bgo_plan_graph_launch (same wait/fetch).
+ * No memset nodes are recorded: counters are cleared by a kernel. */
+int bgo_plan_graph_launch(bgo_plan* p, int32_t strand_mask);
+
+/* Resolved tuning of a plan (after defaults/autotune are applied). */
+int bgo_plan_get_tuning(const bgo_plan* p, bgo_tuning* out);
+
+/* Number of PAM families of a plan (families are numbered in order of first
+ * appearance in the guide array) and, for the first `cap` of them, the number
+ * of PAM-passing windows the last launch compared and the number of guides
+ * (for roofline accounting). */
+int bgo_plan_family_stats(const bgo_plan* p, int32_t cap, int32_t* n_fam_out,
+                          int64_t* windows_out /*[cap]*/,
+                          int32_t* guides_out /*[cap]*/);
+
+/* One-shot convenience: create a plan, scan both strands of the whole
+ * genome, fetch, destroy.  Returns the exact count in n_hits_out; stores
+ * min(n_hits, out_capacity) records. */
+int bgo_scan_batched(bgo_genome* g, const bgo_guide_desc* guides,
+                     int32_t n_guides, const double* score_table,
+                     const bgo_tuning* tuning, bgo_hit* out,
+                     int64_t out_capacity, int64_t* n_hits_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BGO_H */

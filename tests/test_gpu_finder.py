@@ -1,0 +1,128 @@
+"""GPU finder: both engines reproduce the oracle finder's table and the
+reference's known answers."""
+import logging
+
+import numpy as np
+import pandas as pd
+import pytest
+
+import scenarios as S
+from beguidedoff_amd import beguidedoff as make_finder
+from beguidedoff_amd import finder as finder_module
+from beguidedoff_amd import synth
+from beguidedoff_amd.types import GuideInput
+from oracle.finder_oracle import FinderOracle, editable
+from test_host_epilogue import random_case
+
+pytestmark = pytest.mark.gpu
+ENGINES = ["plain", "batched"]
+
+
+def finder(contigs_or_genome, engine, **kw):
+    g = contigs_or_genome if isinstance(contigs_or_genome, dict) else synth.genome_from_strings(contigs_or_genome)
+    return make_finder(genome=g, engine=engine, **kw)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("seed", range(4))
+def test_tables_match_oracle(gpu, engine, seed):
+    genome, guides = random_case(seed)
+    for kw in ({}, {"return_all": True, "count_score_threshold": 5.0}):
+        got = finder(genome, engine, max_mismatches=5).search(guides=guides, verbose=False, **kw)
+        want = FinderOracle(genome, max_mismatches=5).search(guides, **kw)
+        pd.testing.assert_frame_equal(got, want)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_small_config_table(gpu, engine):
+    w = synth.workload("small")
+    got = finder(w.genome, engine, max_mismatches=3).search(guides=w.guides, return_all=True,
+                                                           count_score_threshold=5.0, verbose=False)
+    want = FinderOracle(w.genome, max_mismatches=3).search(w.guides, return_all=True,
+                                                           count_score_threshold=5.0)
+    pd.testing.assert_frame_equal(got, want)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_overflow_rescan_counts_every_site(gpu, engine):
+    f = finder(S.repeat_genome(), engine, max_mismatches=4, n_top_offtargets=3)
+    df = f.search(guides=S.single_guide(), count_score_threshold=5.0, verbose=False)
+    assert len(df) == 3 and df.iloc[0]["n_mismatches"] == 1
+    assert df.iloc[0]["n_offtargets_below_threshold"] == S.N_COPIES
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_cap_truncates_and_warns(gpu, engine, monkeypatch, caplog):
+    f = finder(S.repeat_genome(), engine, max_mismatches=4, n_top_offtargets=3)
+    monkeypatch.setattr(finder_module, "_MAX_RESULTS_PER_GUIDE",
+                        finder_module._DEFAULT_RESULTS_PER_GUIDE)
+    with caplog.at_level(logging.WARNING):
+        df = f.search(guides=S.single_guide(), return_all=True, verbose=False)
+    assert len(df) == finder_module._DEFAULT_RESULTS_PER_GUIDE
+    msgs = [r.getMessage() for r in caplog.records]
+    assert any("Truncated off-target enumeration" in m for m in msgs)
+    assert any(str(S.N_COPIES) in m for m in msgs)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_n_gate_adjacency_and_no_mask(gpu, engine):
+    df = finder(S.n_gate_genome(), engine).search(guides=S.poly_guides(), return_all=True, verbose=False)
+    for _, grp in df.groupby("guide_rna_sequence"):
+        assert len(grp) < 15 and (grp["n_mismatches"] == 0).any()
+        assert not ((grp["ot_position"] >= 104) & (grp["ot_position"] <= 603)).any()
+    df = finder(S.n_adjacent_genome(), engine).search(
+        guides=[GuideInput("polyA", "A" * 20, "NGG", "3prime", 4, 8, "CBE")], return_all=True,
+        verbose=False)
+    assert len(df) == 2 and (df["ot_strand"] == "-").all()
+    z = df[df["n_mismatches"] == 0]
+    assert len(z) == 1 and z.iloc[0]["ot_sequence"] == "A" * 20
+    g = synth.genome_from_strings(S.no_n_mask_genome())
+    for k in ("n_mask", "n_mask_le_fwd", "n_mask_le_rc"):
+        g.pop(k)
+    df = make_finder(genome=g, engine=engine).search(
+        guides=[GuideInput("polyA", "A" * 20, "AAA", "3prime", 4, 8, "CBE")], return_all=True,
+        verbose=False)
+    assert (df["n_mismatches"] == 0).sum() >= 1
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_auto_chunking_2100_keys(gpu, engine):
+    contigs, guides, planted = S.auto_chunk()
+    df = finder(contigs, engine).search(guides=guides, verbose=False)
+    rows = df[df["guide_rna_sequence"] == planted.guide_seq]
+    assert len(rows) == 3 and (rows["n_mismatches"] == 0).all()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_exotic_pams(gpu, engine):
+    contigs, inputs = S.exotic_pam()
+    got = finder(contigs, engine).search(guides=inputs, return_all=True, verbose=False)
+    want = FinderOracle(synth.genome_from_strings(contigs)).search(inputs, return_all=True)
+    pd.testing.assert_frame_equal(got, want)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_positions_ontarget_and_thresholds(gpu, engine):
+    df = finder(S.offtarget_pos31_genome(), engine).search(
+        guides=[GuideInput("g1", S.GUIDE, "NGG", "3prime", 4, 8, "CBE")], verbose=False)
+    assert df["ot_position"].tolist() == [31] and abs(df.iloc[0]["score"] - 1.2 ** 3) < 1e-9
+    ex = GuideInput("g1", S.GUIDE, "NRG", "3prime", 4, 8, "CBE", on_target_chrom="chr1",
+                    on_target_pos=37, on_target_strand="-")
+    df = finder(S.minus_strand_genome(), engine).search(guides=[ex], verbose=False)
+    assert df["ot_position"].tolist() == [83]
+    f = finder(S.threshold_genome(), engine)
+    for thr, count in ((5.0, 3), (3.0, 2), (2.0, 1), (0.5, 0)):
+        df = f.search(guides=[S.threshold_guide()], count_score_threshold=thr, verbose=False)
+        assert df["n_offtargets_below_threshold"].iloc[0] == count
+
+
+def test_tsv_input_and_output(gpu, tmp_path):
+    w = synth.workload("small")
+    tsv = tmp_path / "g.tsv"
+    rows = ["guide_rna_sequence\tbase_editor"] + [f"{g.guide_seq}\tABE8e" for g in w.guides]
+    tsv.write_text("\n".join(rows) + "\n")
+    out = tmp_path / "o.tsv"
+    df = make_finder(genome=w.genome, max_mismatches=3).search(input_file=str(tsv),
+                                                             output_file=str(out), verbose=False)
+    back = pd.read_csv(out, sep="\t")
+    assert len(back) == len(df) and list(back.columns) == list(df.columns)

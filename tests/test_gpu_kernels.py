@@ -1,0 +1,207 @@
+"""GPU parity: the plain drop-in kernel and the batched engine against the
+scan oracle, bit-exact as sorted sets."""
+import numpy as np
+import pytest
+
+from beguidedoff_amd import synth
+from beguidedoff_amd.batched import DeviceGenome, GuideSpec, Plan, guide_descs, scan
+from beguidedoff_amd.cuda_search import MAX_GUIDES, search_genome_cuda
+from beguidedoff_amd.encoding import encode_dna, encode_guide_64bit
+from beguidedoff_amd.finder import cctop_scores
+from beguidedoff_amd.pam import encode_pam_mask
+from oracle.scan import scan_strand
+from test_oracle_pins import build_case
+
+pytestmark = pytest.mark.gpu
+
+STRANDS = ((0, "genome_forward", "n_mask_le_fwd", 0), (1, "genome_rc", "n_mask_le_rc", 3))
+
+
+def arrays(guides):
+    vals = np.array([encode_guide_64bit(encode_dna(s)) & ((1 << 64) - 1) for s, _ in guides],
+                    dtype=np.uint64)
+    masks = np.stack([encode_pam_mask(p) for _, p in guides])
+    lens = np.array([len(p) for _, p in guides], np.uint8)
+    return vals, masks, lens
+
+
+def oracle_set(genome, guides, max_mm, three=None, strands=(0, 1), pos=(0, None), mm_per=None):
+    vals, masks, lens = arrays(guides)
+    tp = np.ones(len(guides), np.uint8) if three is None else three
+    out = set()
+    for s, key, nm, enc in STRANDS:
+        if s not in strands or key not in genome:
+            continue
+        if mm_per is None:
+            g, p, m = scan_strand(genome[key], genome["total_bp"], vals, masks, lens, tp, max_mm,
+                                  genome.get(nm), enc, pos_begin=pos[0], pos_end=pos[1])
+            out |= set(zip(g.tolist(), [s] * g.size, p.tolist(), m.tolist()))
+        else:
+            for gi, k in enumerate(mm_per):
+                g, p, m = scan_strand(genome[key], genome["total_bp"], vals[gi:gi + 1],
+                                      masks[gi:gi + 1], lens[gi:gi + 1], tp[gi:gi + 1], k,
+                                      genome.get(nm), enc, pos_begin=pos[0], pos_end=pos[1])
+                out |= {(gi, s, pp, mm) for pp, mm in zip(p.tolist(), m.tolist())}
+    return out
+
+
+def plain_set(genome, guides, max_mm, three=None, chunk=200_000_000, rpg=10240):
+    vals, masks, lens = arrays(guides)
+    tp = np.ones(len(guides), np.uint8) if three is None else three
+    out = set()
+    for s, key, nm, enc in STRANDS:
+        res = search_genome_cuda(genome[key], genome["total_bp"], vals, masks, lens, tp, max_mm,
+                                 chunk_size=chunk, results_per_guide=rpg,
+                                 n_mask_packed=genome.get(nm), n_encoded=enc)
+        out |= {(g, s, p, m) for g, hits in res.items() for p, m in hits}
+    return out
+
+
+def hit_set(hits):
+    return set(zip(hits["guide"].tolist(), hits["strand"].tolist(), hits["pos"].tolist(),
+                   hits["n_mm"].tolist()))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_plain_matches_oracle_small(gpu, seed):
+    contigs, guides, max_mm = build_case(seed)
+    genome = synth.genome_from_strings(contigs)
+    assert plain_set(genome, guides, max_mm) == oracle_set(genome, guides, max_mm)
+
+
+def test_plain_chunks_five_prime_and_no_mask(gpu):
+    contigs, guides, max_mm = build_case(21, length=(3000, 6000), n_guides=8)
+    genome = synth.genome_from_strings(contigs)
+    three = np.array([1, 0] * 4, np.uint8)
+    want = oracle_set(genome, guides, max_mm, three=three)
+    assert plain_set(genome, guides, max_mm, three=three, chunk=1000) == want
+    nomask = {k: v for k, v in genome.items() if not k.startswith("n_mask")}
+    assert plain_set(nomask, guides, max_mm) == oracle_set(nomask, guides, max_mm)
+
+
+def test_plain_result_buffer_overflow(gpu):
+    contigs = [("chr1", ("T" + "ACGTACGTACGTACGTACGT"[1:] + "AGG") * 500 + "A" * 30)]
+    genome = synth.genome_from_strings(contigs)
+    vals, masks, lens = arrays([("ACGTACGTACGTACGTACGT", "NGG")])
+    full, ovf, cnt = search_genome_cuda(genome["genome_forward"], genome["total_bp"], vals, masks,
+                                        lens, np.ones(1, np.uint8), 4, return_overflow=True,
+                                        results_per_guide=10240)
+    assert not ovf[0] and cnt[0] == len(full[0]) == 500
+    part, ovf, cnt = search_genome_cuda(genome["genome_forward"], genome["total_bp"], vals, masks,
+                                        lens, np.ones(1, np.uint8), 4, return_overflow=True,
+                                        results_per_guide=100)
+    assert ovf[0] and cnt[0] == 500 and len(part[0]) == 100
+    assert set(part[0]) <= set(full[0])
+    with pytest.raises(RuntimeError, match="Too many guides"):
+        search_genome_cuda(genome["genome_forward"], genome["total_bp"],
+                           np.zeros(MAX_GUIDES + 1, np.uint64), np.zeros((MAX_GUIDES + 1, 6), np.uint8),
+                           np.full(MAX_GUIDES + 1, 3, np.uint8), np.ones(MAX_GUIDES + 1, np.uint8), 4)
+
+
+def test_plain_small_config(gpu):
+    w = synth.workload("small")
+    seqs = [g.guide_seq for g in w.guides]
+    guides = [(s, "NGG") for s in seqs]
+    assert plain_set(w.genome, guides, 3) == oracle_set(w.genome, guides, 3)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_batched_matches_oracle_small(gpu, seed):
+    contigs, guides, max_mm = build_case(seed)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"]) if "N" in "".join(
+        s for _, s in contigs) else DeviceGenome(genome)
+    nm = genome if dg.has_n_mask else {k: v for k, v in genome.items() if not k.startswith("n_mask")}
+    r = scan(dg, [GuideSpec(s, p, max_mm) for s, p in guides])
+    assert hit_set(r.hits) == oracle_set(nm, guides, max_mm)
+
+
+def test_batched_mixed_budgets_many_families_ranges(gpu):
+    """>8 PAM families (two launch groups), per-guide budgets, 20/21-nt
+    guides, position ranges and single-strand launches."""
+    rng = np.random.default_rng(3)
+    contigs, guides, _ = build_case(77, length=(20_000, 40_000), n_guides=12)
+    pams = ["NGG", "NRG", "NNGRRT", "NNNRRT", "NG", "TTTV", "SGG", "WGG", "A", "NNNNNN", "BDHV", "GG"]
+    guides = [(s, pams[i]) for i, (s, _) in enumerate(guides)]
+    budgets = [int(rng.integers(0, 6)) for _ in guides]
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
+    r = scan(dg, specs)
+    assert len(r.families) == 12
+    assert hit_set(r.hits) == oracle_set(genome, guides, None, mm_per=budgets)
+    L = genome["total_bp"]
+    for (b, e), strands in [((1000, 17_000), 3), ((0, L // 2), 1), ((L // 3, L), 2)]:
+        r = scan(dg, specs, pos_begin=b, pos_end=e, strands=strands)
+        want = oracle_set(genome, guides, None, mm_per=budgets, pos=(b, e),
+                          strands=[s for s in (0, 1) if strands & (1 << s)])
+        assert hit_set(r.hits) == want
+
+
+def test_batched_capacity_rerun_and_fused_epilogue(gpu):
+    rng = np.random.default_rng(9)
+    contigs, guides, max_mm = build_case(31, length=(50_000, 80_000), n_guides=6, max_mm=5)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, 6, ["ABE", "CBE"][i % 2], (int(rng.integers(1, 6)), int(rng.integers(6, 21))))
+             for i, (s, p) in enumerate(guides)]
+    r = scan(dg, specs, hit_capacity=3)
+    assert r.launches == 2 and r.n_hits == r.hits.size > 3
+    assert hit_set(r.hits) == oracle_set(genome, guides, 6)
+    h = r.hits
+    for s, key in ((0, "genome_forward"), (1, "genome_rc")):
+        sel = h["strand"] == s
+        pos = h["pos"][sel].astype(np.int64)
+        codes = np.stack([synth.extract_codes(genome[key], int(p), 26) for p in pos]) if pos.size else np.zeros((0, 26), np.uint8)
+        b0 = (codes & 1).astype(np.uint32) << np.arange(26, dtype=np.uint32)
+        b1 = (codes >> 1).astype(np.uint32) << np.arange(26, dtype=np.uint32)
+        assert np.array_equal(h["x0"][sel], b0.sum(1).astype(np.uint32))
+        assert np.array_equal(h["x1"][sel], b1.sum(1).astype(np.uint32))
+        gcodes = np.stack([np.pad(encode_dna(specs[g].seq)[:20], (0, 0)) for g in h["guide"][sel]]) if pos.size else np.zeros((0, 20), np.uint8)
+        mm = ((codes[:, :20] != gcodes).astype(np.uint32) << np.arange(20, dtype=np.uint32)).sum(1)
+        assert np.array_equal(h["mm_mask"][sel], mm.astype(np.uint32))
+        assert np.array_equal(h["score"][sel], cctop_scores(mm.astype(np.uint32)))
+        from beguidedoff_amd.base_editing import editable_base_in_window
+        for row, c in zip(h[sel], codes):
+            sp = specs[row["guide"]]
+            seq = "".join("ACGT"[x] for x in c[:20])
+            assert bool(row["editable"]) == bool(editable_base_in_window(seq, sp.editor_type, *sp.window))
+
+
+def test_batched_family_window_counts(gpu):
+    contigs, guides, max_mm = build_case(5, length=(5000, 9000))
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    pams = ["NGG", "NNNRRT", "NG", "TTTV"]
+    specs = [GuideSpec(guides[0][0], p, 3) for p in pams]
+    r = scan(dg, specs)
+    L = genome["total_bp"]
+    from beguidedoff_amd.pam import IUPAC_MASK
+    for (n_g, windows), pam in zip(r.families, pams):
+        want = 0
+        for key in ("genome_forward", "genome_rc"):
+            c = synth.extract_codes(genome[key], 0, L)
+            ok = np.ones(L - 26, bool)
+            for i, ch in enumerate(pam):
+                ok &= ((IUPAC_MASK[ch] >> c[20 + i:20 + i + L - 26]) & 1).astype(bool)
+            want += int(ok.sum())
+        assert n_g == 1 and windows == want
+
+
+def test_graph_replays_match_eager(gpu):
+    contigs, guides, max_mm = build_case(8, length=(30_000, 50_000), n_guides=8)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, max_mm) for s, p in guides]
+    plan = Plan(dg, specs, hit_capacity=1 << 16)
+    plan.launch()
+    n, _ = plan.wait()
+    eager = hit_set(plan.fetch(n))
+    for _ in range(4):
+        plan.graph_launch()
+        n2, _ = plan.wait()
+        assert n2 == n and hit_set(plan.fetch(n2)) == eager
+        plan.launch(0, genome["total_bp"] // 2, 1)   # eager work between replays
+        plan.wait()
+    plan.close()
+    assert eager == oracle_set(genome, guides, max_mm)

@@ -1,0 +1,102 @@
+"""The reference's known-answer scenarios, checked on the finder oracle (CPU).
+The same scenarios run on the GPU product in test_gpu_finder.py."""
+import numpy as np
+import pytest
+
+import scenarios as S
+from beguidedoff_amd import synth
+from beguidedoff_amd.types import GuideInput
+from brute import contig_sites
+from oracle.finder_oracle import FinderOracle, editable
+
+
+def F(contigs, **kw):
+    return FinderOracle(synth.genome_from_strings(contigs), **kw)
+
+
+def test_repeat_genome_counts_every_site():
+    df = F(S.repeat_genome(), max_mismatches=4).search(S.single_guide(), count_score_threshold=5.0)
+    assert len(df) == 3
+    assert df.iloc[0]["n_mismatches"] == 1
+    assert df.iloc[0]["n_offtargets_below_threshold"] == S.N_COPIES
+
+
+def test_n_gate():
+    df = F(S.n_gate_genome()).search(S.poly_guides(), return_all=True)
+    for _, grp in df.groupby("guide_rna_sequence"):
+        assert len(grp) < 15
+        assert not ((grp["ot_position"] >= 104) & (grp["ot_position"] <= 603)).any()
+        assert (grp["n_mismatches"] == 0).any()
+
+
+def test_site_adjacent_to_n_run():
+    df = F(S.n_adjacent_genome()).search(
+        [GuideInput("polyA", "A" * 20, "NGG", "3prime", 4, 8, "CBE")], return_all=True)
+    assert len(df) == 2 and (df["ot_strand"] == "-").all()
+    z = df[df["n_mismatches"] == 0]
+    assert len(z) == 1 and z.iloc[0]["ot_sequence"] == "A" * 20
+
+
+def test_genome_without_n_mask():
+    g = synth.genome_from_strings(S.no_n_mask_genome())
+    g.pop("n_mask")
+    g.pop("n_mask_le_fwd")
+    g.pop("n_mask_le_rc")
+    df = FinderOracle(g).search([GuideInput("polyA", "A" * 20, "AAA", "3prime", 4, 8, "CBE")],
+                                return_all=True)
+    assert (df["n_mismatches"] == 0).sum() >= 1
+
+
+def test_auto_chunk_planted_guide():
+    contigs, guides, planted = S.auto_chunk()
+    df = F(contigs).search(guides)
+    rows = df[df["guide_rna_sequence"] == planted.guide_seq]
+    assert len(rows) == 3 and (rows["n_mismatches"] == 0).all()
+
+
+def test_exotic_pams_match_bruteforce():
+    contigs, inputs = S.exotic_pam()
+    df = F(contigs).search(inputs, return_all=True)
+    et = {g.guide_seq: g.editor_type for g in inputs}
+    got = {(int(r.ot_position), r.ot_strand, int(r.n_mismatches), round(float(r.score), 6))
+           for r in df.itertuples() if r.score <= 5.0
+           and editable(r.ot_sequence, et[r.guide_rna_sequence], 4, 8)}
+    ref = set()
+    for gi, name, pos, strand, mm in contig_sites(contigs, [(g.guide_seq, g.pam) for g in inputs], 4):
+        pass
+    # score/editability filter of tests/test_extensive_gpu.py:268-299
+    from oracle.finder_oracle import cctop
+    seq = contigs[0][1]
+    from brute import revcomp
+    for g in inputs:
+        win = 20 + len(g.pam)
+        for gi, name, pos, strand, mm in contig_sites(contigs, [(g.guide_seq, g.pam)], 4):
+            if strand == "+":
+                w = seq[pos - 1:pos - 1 + 20]
+            else:
+                left = pos - win
+                w = revcomp(seq[left:left + win])[:20]
+            mmpos = [i + 1 for i in range(20) if w[i] != g.guide_seq[i]]
+            sc = cctop(mmpos)
+            if sc <= 5.0 and editable(w, "CBE", 4, 8):
+                ref.add((pos, strand, mm, round(sc, 6)))
+    assert got == ref and len(ref) >= 8
+
+
+def test_search_positions():
+    g = [GuideInput("g1", S.GUIDE, "NGG", "3prime", 4, 8, "CBE")]
+    df = F(S.offtarget_pos31_genome()).search(g)
+    assert len(df) == 1 and df.iloc[0]["ot_position"] == 31 and df.iloc[0]["n_mismatches"] == 1
+    assert abs(df.iloc[0]["score"] - 1.2 ** 3) < 1e-9
+    df = F(S.minus_strand_genome()).search([GuideInput("g1", S.GUIDE, "NRG", "3prime", 4, 8, "CBE")])
+    assert sorted(df["ot_position"].tolist()) == [37, 83] and (df["ot_strand"] == "-").all()
+    ex = GuideInput("g1", S.GUIDE, "NRG", "3prime", 4, 8, "CBE", on_target_chrom="chr1",
+                    on_target_pos=37, on_target_strand="-")
+    df = F(S.minus_strand_genome()).search([ex])
+    assert df["ot_position"].tolist() == [83]
+
+
+@pytest.mark.parametrize("thr,count", [(5.0, 3), (3.0, 2), (2.0, 1), (0.5, 0)])
+def test_count_score_threshold(thr, count):
+    df = F(S.threshold_genome()).search([S.threshold_guide()], count_score_threshold=thr)
+    assert len(df) == 3 and df["n_offtargets_below_threshold"].iloc[0] == count
