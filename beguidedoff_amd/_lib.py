@@ -39,6 +39,8 @@ _SIGNATURES = {
     "bgo_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
     "bgo_init": (C.c_int, [C.c_int32]),
     "bgo_last_error": (C.c_char_p, []),
+    "bgo_device_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int64), C.c_char_p, C.c_int32]),
     "bgo_debug_fail_next": (C.c_int, [C.c_int32]),
     "bgo_strand_create": (C.c_int, [_P, C.c_int64, _P, C.c_int64, C.c_int64, C.POINTER(_P)]),
     "bgo_strand_destroy": (C.c_int, [_P]),
@@ -104,6 +106,14 @@ def device_count() -> int:
     n = C.c_int32(0)
     check(load().bgo_device_count(C.byref(n)))
     return int(n.value)
+
+
+def device_info(device: int = 0) -> dict:
+    cu, clk, mem = C.c_int32(0), C.c_int32(0), C.c_int64(0)
+    arch = C.create_string_buffer(64)
+    check(load().bgo_device_info(int(device), C.byref(cu), C.byref(clk), C.byref(mem), arch, 64))
+    return {"cu_count": int(cu.value), "clock_khz": int(clk.value), "hbm_bytes": int(mem.value),
+            "arch": arch.value.decode()}
 
 
 _inited: set = set()

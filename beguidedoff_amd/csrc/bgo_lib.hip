@@ -36,3 +36,17 @@ extern "C" int bgo_init(int32_t device) {
                      prop.gcnArchName);
   return BGO_OK;
 }
+
+extern "C" int bgo_device_info(int32_t device, int32_t* cu_count, int32_t* clock_khz,
+                               int64_t* hbm_bytes, char* arch, int32_t arch_len) {
+  BGO_TRY(bgo::injected());
+  hipDeviceProp_t prop;
+  BGO_HIP(hipGetDeviceProperties(&prop, device));
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (clock_khz) *clock_khz = prop.clockRate;
+  if (hbm_bytes) *hbm_bytes = static_cast<int64_t>(prop.totalGlobalMem);
+  if (arch && arch_len > 0) {
+    std::snprintf(arch, static_cast<size_t>(arch_len), "%s", prop.gcnArchName);
+  }
+  return BGO_OK;
+}

@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the extensive-mode off-target scan on one MI355X.
+
+Workload (default): the 3.09 Gbp hg38-like synthetic genome (24 contigs,
+41% GC, ~5% N) and 1024 mixed guides (512 NGG/20 nt, 256 NNNRRT/21 nt,
+128 NNGRRT/21 nt, 128 NG/20 nt; half ABE 4-8, half CBE 3-9) at <= 4 mm.
+
+A step is one full scan of both strands for every guide by the batched
+persistent kernel (hipGraph replay) including the copy of all site records
+to the host; the genome and guide table are resident in HBM before timing.
+The plain drop-in kernel (the reference's decomposition) is timed on the
+same workload, and the oracle (CPU restatement) on a bounded sample of it.
+
+Prints one JSON line.  guide*Gbp/s = guides x single-strand genome Gbp /
+seconds, both strands always scanned.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "guide·Gbp/s at ≤4 mm, batched vs plain kernel; p50/p99 request latency"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def device_peaks(info: dict) -> dict:
+    clk = info["clock_khz"] * 1e3 if info["clock_khz"] > 0 else 2.4e9
+    return {"valu_int32_ops": info["cu_count"] * 64 * clk, "clock_hz": clk,
+            "hbm_bytes_per_s": 8.0e12}
+
+
+def algorithmic_ops(L_pos: int, n_strands: int, families) -> tuple:
+    """(ops of the bit-plane formulation, ops of the per-unit figures used in
+    the reference survey) for one launch; families = [(guides, windows)]
+    where windows counts PAM-passing windows over both strands."""
+    ops = n_strands * L_pos * 2                       # window + PAM extraction
+    ops += sum(n_strands * L_pos * 4 for _ in families)  # 3 bit-selects + compare
+    ops += sum(4 * g * w for g, w in families)        # xor, bitop3, bcnt, test
+    survey = sum(n_strands * L_pos * 3 + 11 * g * w for g, w in families)
+    return ops, survey
+
+
+def cpu_baseline(workload, max_mm: int, target_s: float = 12.0) -> dict:
+    from oracle.scan import scan_strand
+    from beguidedoff_amd.encoding import encode_dna, encode_guide_64bit
+    from beguidedoff_amd.pam import encode_pam_mask
+
+    g = workload.genome
+    guides = workload.guides
+    vals = np.array([encode_guide_64bit(encode_dna(x.guide_seq)) & ((1 << 64) - 1) for x in guides],
+                    dtype=np.uint64)
+    masks = np.stack([encode_pam_mask(x.pam) for x in guides])
+    lens = np.array([len(x.pam) for x in guides], np.uint8)
+    tp = np.ones(len(guides), np.uint8)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 64))
+    start = int(g["chrom_offsets"][0]) + 1_000_000
+
+    def run(n_pos):
+        t = time.perf_counter()
+        for key, nm, enc in (("genome_forward", "n_mask_le_fwd", 0), ("genome_rc", "n_mask_le_rc", 3)):
+            scan_strand(g[key], g["total_bp"], vals, masks, lens, tp, max_mm, g.get(nm), enc,
+                        pos_begin=start, pos_end=start + n_pos, threads=threads)
+        return time.perf_counter() - t
+
+    probe = 20_000
+    dt = run(probe)
+    n_pos = int(min(max(probe, probe * target_s / max(dt, 1e-6)), 50_000_000))
+    dt = run(n_pos)
+    value = len(guides) * (n_pos / 1e9) / dt
+    return {"value": value, "unit": "guide·Gbp/s", "cores": threads, "kind": "port",
+            "sample": f"oracle scan of all {len(guides)} guides, both strands, {n_pos} positions "
+                      f"({dt:.1f} s, {threads} threads)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="hg38_mixed")
+    ap.add_argument("--max-mm", type=int, default=4)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default="", help="JSON dict of bgo_tuning fields")
+    ap.add_argument("--eager", action="store_true", help="launch without the hipGraph")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        print("bench.py: only --gpus 1 is supported (one MI355X per box)", file=sys.stderr)
+        return 2
+
+    from beguidedoff_amd import _lib, synth
+    from beguidedoff_amd.batched import DeviceGenome, GuideSpec, Plan, default_capacity
+
+    _lib.init(0)
+    info = _lib.device_info(0)
+    peaks = device_peaks(info)
+    log(f"device {info}")
+
+    t0 = time.perf_counter()
+    w = synth.workload(args.workload, scale=args.scale)
+    g = w.genome
+    L = int(g["total_bp"])
+    log(f"workload {w.name}: {L} bp, {len(w.guides)} guides, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+
+    specs = [GuideSpec(x.guide_seq, x.pam, args.max_mm, x.editor_type,
+                       (x.edit_window_start, x.edit_window_end)) for x in w.guides]
+    t0 = time.perf_counter()
+    dg = DeviceGenome(g, g["n_mask_le_fwd"], g["n_mask_le_rc"])
+    log(f"genome resident in HBM ({dg.device_bytes() / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s")
+    tuning = json.loads(args.tuning) if args.tuning else None
+    plan = Plan(dg, specs, tuning=tuning, hit_capacity=default_capacity(len(specs)))
+
+    def step():
+        if args.eager:
+            plan.launch()
+        else:
+            plan.graph_launch()
+        n, ms = plan.wait()
+        if n > plan.hit_capacity:
+            raise RuntimeError(f"hit buffer too small ({n} > {plan.hit_capacity})")
+        hits = plan.fetch(n)
+        return n, ms, hits
+
+    for i in range(args.warmup):
+        n, ms, _ = step()
+        log(f"warmup {i}: {n} sites, kernel {ms:.2f} ms")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        n, ms, hits = step()
+        kernel_ms.append(ms)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    gbp = L / 1e9
+    value = len(specs) * gbp / (ms_per_step / 1e3)
+    kms = float(np.mean(kernel_ms))
+    families = plan.family_stats()
+    L_pos = max(0, L - 26)
+    ops, survey_ops = algorithmic_ops(L_pos, 2, families)
+    achieved = ops / (kms / 1e3) / 1e12
+    peak = peaks["valu_int32_ops"] / 1e12
+    log(f"batched: {ms_per_step:.2f} ms/step (kernel {kms:.2f} ms), {value:.1f} guide·Gbp/s, "
+        f"{n} sites, families {families}")
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "guide·Gbp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "description": w.description, "genome_bp": L,
+                   "guides": len(specs), "max_mismatches": args.max_mm,
+                   "pams": sorted({s.pam for s in specs}), "engine": "batched VALU",
+                   "graph": not args.eager, "tuning": plan.tuning()},
+        "sites": int(n),
+        "kernel_ms": round(kms, 3),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
+                     "unit": "Tops/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "ops_per_launch": int(ops), "survey_ops_per_launch": int(survey_ops),
+                     "survey_frac": round(survey_ops / (kms / 1e3) / peaks["valu_int32_ops"], 4),
+                     "hbm_gbps_planes": round(2 * L / 4 / (kms / 1e3) / 1e9, 1)},
+    }
+
+    if not args.no_plain:
+        from beguidedoff_amd.cuda_search import last_kernel_ms, search_genome_cuda
+        from beguidedoff_amd.encoding import encode_dna, encode_guide_64bit
+        from beguidedoff_amd.pam import encode_pam_mask
+
+        vals = np.array([encode_guide_64bit(encode_dna(s.seq)) & ((1 << 64) - 1) for s in specs],
+                        dtype=np.uint64)
+        masks = np.stack([encode_pam_mask(s.pam) for s in specs])
+        lens = np.array([len(s.pam) for s in specs], np.uint8)
+        tp = np.ones(len(specs), np.uint8)
+        plain_ms, plain_wall, plain_sites = [], [], 0
+        for rep in range(args.plain_steps):
+            kt, tw, sites = 0.0, time.perf_counter(), 0
+            for key, nm, enc in (("genome_forward", "n_mask_le_fwd", 0),
+                                 ("genome_rc", "n_mask_le_rc", 3)):
+                res, ovf, cnt = search_genome_cuda(g[key], L, vals, masks, lens, tp, args.max_mm,
+                                                   return_overflow=True, n_mask_packed=g[nm],
+                                                   n_encoded=enc)
+                kt += last_kernel_ms()
+                sites += sum(cnt.values())
+            plain_wall.append(time.perf_counter() - tw)
+            plain_ms.append(kt)
+            plain_sites = sites
+            log(f"plain rep {rep}: kernels {kt:.1f} ms, wall {plain_wall[-1]:.1f} s, {sites} sites")
+        pk = float(np.mean(plain_ms))
+        out["plain"] = {"value": round(len(specs) * gbp / (pk / 1e3), 2), "kernel_ms": round(pk, 2),
+                        "wall_s": round(float(np.mean(plain_wall)), 2), "sites": int(plain_sites),
+                        "steps": args.plain_steps}
+        out["speedup_vs_plain"] = round(pk / kms, 2)
+        out["sites_equal_plain"] = bool(plain_sites == n)
+
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(w, args.max_mm)
+        log(f"cpu baseline {out['cpu_baseline']} in {time.perf_counter() - t0:.1f} s")
+
+    plan.close()
+    dg.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

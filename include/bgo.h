@@ -50,6 +50,11 @@ int bgo_device_count(int32_t* n_out);
  * library's per-device state.  Idempotent. */
 int bgo_init(int32_t device);
 
+/* Properties of a device: compute units, peak engine clock (kHz), HBM bytes
+ * and the gfx architecture name (NUL-terminated, truncated to arch_len). */
+int bgo_device_info(int32_t device, int32_t* cu_count, int32_t* clock_khz,
+                    int64_t* hbm_bytes, char* arch, int32_t arch_len);
+
 /* Message of the last failed call on this thread ("" when none). */
 const char* bgo_last_error(void);
 

@@ -34,13 +34,18 @@ def test_tables_match_oracle(gpu, engine, seed):
 
 
 @pytest.mark.parametrize("engine", ENGINES)
-def test_small_config_table(gpu, engine):
+@pytest.mark.parametrize("max_mm", [3, 5])
+def test_small_config_table(gpu, engine, max_mm):
+    """3 contigs (10 Mbp, 1% N), 8 random NGG guides: at <= 3 mm random
+    guides rarely have sites; at <= 5 mm the table has dozens of rows."""
     w = synth.workload("small")
-    got = finder(w.genome, engine, max_mismatches=3).search(guides=w.guides, return_all=True,
-                                                           count_score_threshold=5.0, verbose=False)
-    want = FinderOracle(w.genome, max_mismatches=3).search(w.guides, return_all=True,
-                                                           count_score_threshold=5.0)
+    got = finder(w.genome, engine, max_mismatches=max_mm).search(
+        guides=w.guides, return_all=True, count_score_threshold=5.0, verbose=False)
+    want = FinderOracle(w.genome, max_mismatches=max_mm).search(
+        w.guides, return_all=True, count_score_threshold=5.0)
     pd.testing.assert_frame_equal(got, want)
+    if max_mm == 5:
+        assert len(got) > 10
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -122,7 +127,8 @@ def test_tsv_input_and_output(gpu, tmp_path):
     rows = ["guide_rna_sequence\tbase_editor"] + [f"{g.guide_seq}\tABE8e" for g in w.guides]
     tsv.write_text("\n".join(rows) + "\n")
     out = tmp_path / "o.tsv"
-    df = make_finder(genome=w.genome, max_mismatches=3).search(input_file=str(tsv),
+    df = make_finder(genome=w.genome, max_mismatches=5).search(input_file=str(tsv),
                                                              output_file=str(out), verbose=False)
+    assert len(df) > 0 and set(df["base_editor"]) == {"ABE8e"}
     back = pd.read_csv(out, sep="\t")
     assert len(back) == len(df) and list(back.columns) == list(df.columns)
