@@ -321,25 +321,25 @@ def workload(name: str, seed: int = 0, scale: float = 1.0) -> Workload:
         seqs = random_guides(np.random.default_rng(0), 8)
         guides = _guide_inputs(seqs, ["NGG"] * 8, [("ABE", 4, 8)] * 8)
         genome = make_genome(contigs, seed=seed, n_frac=0.01, n_len=(1000, 10000))
-        return Workload(name, genome, guides, 3, "3 contigs 10 Mbp, 8 NGG guides, <=3 mm")
+        return Workload(name, genome, guides, 3, "3 contigs 10 Mbp, 8 NGG guides")
     contigs = [(n, max(1000, int(l * scale))) for n, l in HG38_LIKE_CONTIGS]
     names, offsets, lengths = contig_layout(contigs)
-    n_runs = place_n_runs(np.random.default_rng([seed, 1]), offsets, lengths, 0.05,
-                          max(10, int(10_000 * min(1.0, scale * 10))),
-                          max(20, int(1_000_000 * scale)))
+    lo = max(10, int(10_000 * scale))
+    n_runs = place_n_runs(np.random.default_rng([seed, 1]), offsets, lengths, 0.05, lo,
+                          max(lo + 1, int(1_000_000 * scale)))
     if name == "hg38_ngg":
         seqs = random_guides(rng, 128)
         guides = _guide_inputs(seqs, ["NGG"] * 128, [("ABE", 4, 8)] * 128)
         plants = plan_plants(rng, offsets, lengths, *n_runs, [(s, "NGG") for s in seqs], 40, 4)
         genome = make_genome(contigs, seed=seed, n_runs=n_runs, plants=plants)
-        return Workload(name, genome, guides, 4, "hg38-like 3.09 Gbp, 128 NGG guides, <=4 mm")
+        return Workload(name, genome, guides, 4, "hg38-like 3.09 Gbp, 128 NGG guides + 40 plants each")
     if name == "hg38_mixed":
         guides = mixed_guides(rng, 512, 256, 128, 128)
         genome = make_genome(contigs, seed=seed, n_runs=n_runs)
-        return Workload(name, genome, guides, 5, "hg38-like 3.09 Gbp, 1024 mixed-PAM guides, <=5 mm")
+        return Workload(name, genome, guides, 5, "hg38-like 3.09 Gbp, 1024 mixed-PAM guides")
     if name == "hg38_8k":
         seqs = random_guides(rng, 8192)
         guides = _guide_inputs(seqs, ["NGG"] * 8192, [("ABE", 4, 8)] * 8192)
         genome = make_genome(contigs, seed=seed, n_runs=n_runs)
-        return Workload(name, genome, guides, 4, "hg38-like 3.09 Gbp, 8192 NGG guides, <=4 mm")
+        return Workload(name, genome, guides, 4, "hg38-like 3.09 Gbp, 8192 NGG guides")
     raise ValueError(f"unknown workload {name!r}")
