@@ -29,7 +29,9 @@ class Tuning(C.Structure):
                 ("guide_unroll", C.c_int32), ("reserved", C.c_int32 * 3)]
 
     def as_dict(self) -> dict:
-        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
+        d = {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
+        d["interleave"] = int(self.reserved[0])
+        return d
 
 
 # Every symbol of include/bgo.h with its ctypes signature.
@@ -54,6 +56,8 @@ _SIGNATURES = {
     "bgo_plan_create": (C.c_int, [_P, _P, C.c_int32, _P, C.POINTER(Tuning), C.c_int64,
                                   C.POINTER(_P)]),
     "bgo_plan_destroy": (C.c_int, [_P]),
+    "bgo_plan_set_guides": (C.c_int, [_P, _P, C.c_int32]),
+    "bgo_plan_set_hit_capacity": (C.c_int, [_P, C.c_int64]),
     "bgo_plan_launch": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int32]),
     "bgo_plan_wait": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_float)]),
     "bgo_plan_fetch": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64)]),

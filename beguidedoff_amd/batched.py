@@ -61,6 +61,9 @@ def _tuning_struct(tuning: Optional[dict]) -> Optional[_lib.Tuning]:
     for k, v in tuning.items():
         if k == "reserved":
             continue
+        if k == "interleave":
+            t.reserved[0] = int(v)
+            continue
         setattr(t, k, int(v))
     return t
 
@@ -145,6 +148,18 @@ class Plan:
         if self._h is None:
             raise RuntimeError("plan released")
         return self._h
+
+    def set_guides(self, specs: Sequence[GuideSpec]) -> None:
+        """Swap the guide set (waits for the previous launch)."""
+        self.specs = list(specs)
+        self.n_guides = len(self.specs)
+        self.descs = guide_descs(self.specs)
+        _lib.check(_lib.load().bgo_plan_set_guides(self.handle, _lib.ptr(self.descs),
+                                                   self.n_guides))
+
+    def set_hit_capacity(self, capacity: int) -> None:
+        _lib.check(_lib.load().bgo_plan_set_hit_capacity(self.handle, int(capacity)))
+        self.hit_capacity = int(capacity)
 
     def launch(self, pos_begin: int = 0, pos_end: Optional[int] = None, strands: int = 3) -> None:
         end = self.genome.total_bp if pos_end is None else int(pos_end)

@@ -53,10 +53,11 @@ constexpr int kMaxBlock = 256; // threads per block (4 waves)
 constexpr uint32_t kWin = 0xFFFFFu;
 
 struct Fam {
-  uint32_t mA, mC, mG, mT;  // allowed-base masks at bits 20..25, all other bits 1
+  uint32_t nibs;            // allowed-base nibble of PAM position i at bits 4i..4i+3
+  uint32_t plen;            // PAM length 1..6
   uint32_t g_begin, g_end;  // padded guide range
   uint32_t kmax;            // largest budget among the family's guides
-  uint32_t pad;
+  uint32_t pad[3];
 };
 
 struct Args {
@@ -67,6 +68,7 @@ struct Args {
   uint32_t slot_strand0, slot_strand1;  // strand scanned by enumeration slot 0/1
   uint32_t valid_begin, valid_end;
   uint32_t chunk, chunks_per_slot, n_items, n_fam;
+  uint32_t n_slots, interleave;  // interleave: consecutive items alternate strands
   unsigned long long hit_cap;
 };
 
@@ -86,6 +88,12 @@ struct Tab {
 };
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Bytes of LDS one wave uses with nf families (see the kernel's layout).
+__host__ __device__ constexpr size_t wave_lds_bytes(int nf) {
+  return static_cast<size_t>(nf) * kList * 8 + kMaxFam * 8 + static_cast<size_t>(nf) * 64 * 4 +
+         2 * kMaxFam * 4;
+}
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -153,9 +161,10 @@ __device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s
 // Compare the first n (<= 64) windows of a family list with all guides of
 // the family.  One window per lane; guides are wave-uniform.
 __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, uint32_t s,
-                                    const uint2* L, uint32_t n, uint32_t cs, uint32_t lane) {
+                                    const uint2* L, uint32_t head, uint32_t n, uint32_t cs,
+                                    uint32_t lane) {
   if (lane >= n) return;
-  const uint2 e = L[lane];
+  const uint2 e = L[(head + lane) & (kList - 1)];
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
   const uint32_t pos = cs + ((e.x >> 20) | ((e.y >> 20) << 12));
   const uint32_t kmax = F.kmax;
@@ -187,71 +196,142 @@ __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, u
   }
 }
 
+// Bases of a PAM position whose 2-bit code (x1 x0) is in the allowed set
+// `nib` (bit b = code b allowed), for 32 positions at once.
+__device__ inline uint32_t allowed(uint32_t nib, uint32_t x0, uint32_t x1) {
+  switch (nib) {
+    case 0x1: return ~(x0 | x1);      // A
+    case 0x2: return x0 & ~x1;        // C
+    case 0x3: return ~x1;             // M = A|C
+    case 0x4: return ~x0 & x1;        // G
+    case 0x5: return ~x0;             // R = A|G
+    case 0x6: return x0 ^ x1;         // S = C|G
+    case 0x7: return ~(x0 & x1);      // V = not T
+    case 0x8: return x0 & x1;         // T
+    case 0x9: return ~(x0 ^ x1);      // W = A|T
+    case 0xA: return x0;              // Y = C|T
+    case 0xB: return x0 | ~x1;        // H = not G
+    case 0xC: return x1;              // K = G|T
+    case 0xD: return ~x0 | x1;        // D = not C
+    case 0xE: return x0 | x1;         // B = not A
+    case 0xF: return ~0u;             // N
+    default: return 0u;
+  }
+}
+
+// Stream positions [cs, ce) of strand s.  Each lane owns a 32-position block:
+// the PAM test of every family runs on whole plane words (one allowed-set op
+// per PAM position for 32 positions), then the passing windows are appended
+// to the family's ring buffer in LDS; every 64 buffered windows are compared
+// against the family's guides (one window per lane).
+// Per-wave LDS state of the scan loop.
+struct WaveState {
+  uint2* lists;                  // [NF][kList] ring buffers of windows
+  uint32_t* pws;                 // [NF][64] pass words of the current blocks
+  uint32_t* heads;               // [kMaxFam] consumed ring index
+  uint32_t* tails;               // [kMaxFam] appended ring index
+  unsigned long long* wstats;    // [kMaxFam] windows compared
+};
+
+// Append this lane's passing windows (bits of pw in block bb) to family f's
+// ring buffer, comparing every 64 buffered windows.  All lanes call this.
+__device__ inline void append_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
+                                     uint32_t s, uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
+                                     uint32_t cs, uint32_t lane) {
+  uint2* L = w.lists + f * kList;
+  const uint32_t n = __popc(pw);
+  const uint32_t base = atomicAdd(&w.tails[f], n);
+  wave_sync();
+  const uint32_t tail = uniform(w.tails[f]);
+  uint32_t head = uniform(w.heads[f]);
+  uint32_t k = 0;
+  while (true) {
+    const uint32_t limit = head + kList;
+    while (k < n && base + k < limit) {
+      const uint32_t j = __builtin_ctz(pw);
+      pw &= pw - 1;
+      const uint32_t off = (bb << 5) + j - cs;
+      const uint32_t W0 = funnel(qb.x, qa.x, j) & kWin;
+      const uint32_t W1 = funnel(qb.y, qa.y, j) & kWin;
+      L[(base + k) & (kList - 1)] = make_uint2(W0 | (off << 20), W1 | ((off >> 12) << 20));
+      ++k;
+    }
+    wave_sync();
+    if (tail - head < 64) break;
+    compare_list(a, t, t.fam[f], s, L, head, 64, cs, lane);
+    head += 64;
+    if (lane == 0) w.wstats[f] += 64;
+    wave_sync();
+  }
+  if (lane == 0) w.heads[f] = head;
+  wave_sync();
+}
+
+// Stream positions [cs, ce) of strand s.  Each lane owns a 32-position block:
+// the PAM test of every family runs on whole plane words (one allowed-set op
+// per PAM position for 32 positions), then the passing windows are appended
+// to the family's ring buffer in LDS; every 64 buffered windows are compared
+// against the family's guides (one window per lane).
 template <int NF>
-__device__ inline void scan_run(const Args& a, const Tab& t, uint32_t s, uint32_t cs, uint32_t ce,
-                                uint2* lists, unsigned long long* wstats, uint32_t lane) {
+__device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w, uint32_t s,
+                                uint32_t cs, uint32_t ce, uint32_t lane) {
   const uint2* __restrict__ planes = s ? a.planes1 : a.planes0;
-  constexpr uint32_t nf = NF;
-  // Family masks, wave-uniform for the whole run.
-  uint32_t mA[NF], mC[NF], mG[NF], mT[NF];
+  uint32_t nibs[NF], plen[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    mA[f] = t.fam[f].mA;
-    mC[f] = t.fam[f].mC;
-    mG[f] = t.fam[f].mG;
-    mT[f] = t.fam[f].mT;
+    nibs[f] = t.fam[f].nibs;
+    plen[f] = t.fam[f].plen;
   }
-  uint64_t cnts = 0;  // 8-bit fill level per family list
-  for (uint32_t base = cs; base < ce; base += 64) {
-    const uint32_t p = base + lane;
-    const bool valid = p < ce;
-    const uint32_t pp = valid ? p : base;
-    const uint32_t w = pp >> 5, sh = pp & 31u;
-    const uint2 qa = planes[w], qb = planes[w + 1];
-    const uint32_t X0 = funnel(qb.x, qa.x, sh);
-    const uint32_t X1 = funnel(qb.y, qa.y, sh);
-    const uint32_t off = pp - cs;
-    const uint2 entry = make_uint2((X0 & kWin) | (off << 20), (X1 & kWin) | ((off >> 12) << 20));
-    bool full = false;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      // IUPAC test of the 6 PAM bases at once: select the allowed bit of
-      // each base's code from the four per-base masks.
-      const uint32_t lo = (X0 & mC[f]) | (~X0 & mA[f]);
-      const uint32_t hi = (X0 & mT[f]) | (~X0 & mG[f]);
-      const uint32_t Y = (X1 & hi) | (~X1 & lo);
-      const bool pass = valid && (Y == 0xFFFFFFFFu);
-      const uint64_t bal = ballot(pass);
-      if (bal) {
-        uint32_t c = static_cast<uint32_t>(cnts >> (8 * f)) & 0xFFu;
-        if (pass) lists[f * kList + c + lanes_below(bal)] = entry;
-        c += static_cast<uint32_t>(__popcll(bal));
-        cnts = (cnts & ~(0xFFull << (8 * f))) | (static_cast<uint64_t>(c) << (8 * f));
-        full |= (c >= 64);
-      }
-    }
-    if (full) {
-      wave_sync();
-      for (uint32_t f = 0; f < nf; ++f) {
-        uint32_t c = static_cast<uint32_t>(cnts >> (8 * f)) & 0xFFu;
-        if (c < 64) continue;
-        uint2* L = lists + f * kList;
-        compare_list(a, t, t.fam[f], s, L, 64, cs, lane);
-        wave_sync();
-        if (lane < c - 64) L[lane] = L[64 + lane];
-        wave_sync();
-        c -= 64;
-        cnts = (cnts & ~(0xFFull << (8 * f))) | (static_cast<uint64_t>(c) << (8 * f));
-        if (lane == 0) wstats[f] += 64;
-      }
-    }
+  if (lane < kMaxFam) {
+    w.tails[lane] = 0;
+    w.heads[lane] = 0;
   }
   wave_sync();
-  for (uint32_t f = 0; f < nf; ++f) {
-    const uint32_t c = static_cast<uint32_t>(cnts >> (8 * f)) & 0xFFu;
-    if (!c) continue;
-    compare_list(a, t, t.fam[f], s, lists + f * kList, c, cs, lane);
-    if (lane == 0) wstats[f] += c;
+  const uint32_t b_first = cs >> 5, b_last = (ce - 1) >> 5;
+  for (uint32_t b0 = b_first; b0 <= b_last; b0 += 64) {
+    const uint32_t b = b0 + lane;
+    const bool inb = b <= b_last;
+    const uint32_t bb = inb ? b : b_first;
+    const uint2 qa = planes[bb], qb = planes[bb + 1];
+    uint32_t vmask = 0;
+    if (inb) {
+      const uint32_t p0 = bb << 5;
+      const uint32_t lo = cs > p0 ? cs - p0 : 0u;
+      const uint32_t hi = ce - p0 >= 32 ? 32u : ce - p0;
+      vmask = (hi == 32 ? ~0u : ((1u << hi) - 1u)) & (~0u << lo);
+    }
+    // plane bits of PAM position i for the block's 32 positions
+    uint32_t f0[6], f1[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      f0[i] = funnel(qb.x, qa.x, 20 + i);
+      f1[i] = funnel(qb.y, qa.y, 20 + i);
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      uint32_t pw = vmask;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const uint32_t nib = (nibs[f] >> (4 * i)) & 0xFu;
+        if (i < static_cast<int>(plen[f]) && nib != 0xFu) pw &= allowed(nib, f0[i], f1[i]);
+      }
+      w.pws[f * 64 + lane] = pw;
+    }
+    wave_sync();
+#pragma nounroll
+    for (uint32_t f = 0; f < NF; ++f) {
+      const uint32_t pw = w.pws[f * 64 + lane];
+      if (ballot(pw != 0)) append_family(a, t, w, f, s, pw, bb, qa, qb, cs, lane);
+    }
+  }
+#pragma nounroll
+  for (uint32_t f = 0; f < NF; ++f) {
+    const uint32_t head = uniform(w.heads[f]);
+    const uint32_t c = uniform(w.tails[f]) - head;
+    if (c) {
+      compare_list(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
+      if (lane == 0) w.wstats[f] += c;
+    }
   }
   wave_sync();
 }
@@ -269,24 +349,37 @@ __global__ __launch_bounds__(kMaxBlock) void batched_valu_kernel(
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t n_waves = blockDim.x >> 6;
-  const uint32_t per_wave = NF * kList;
-  uint2* lists = lds + wave * per_wave;
-  auto* wstats = reinterpret_cast<unsigned long long*>(lds + n_waves * per_wave) + wave * kMaxFam;
-  if (lane < kMaxFam) wstats[lane] = 0;
+  // LDS per wave: NF*kList windows (8 B), kMaxFam stats (8 B), NF*64 pass
+  // words, kMaxFam heads and tails (4 B each).
+  char* my = reinterpret_cast<char*>(lds) + wave * wave_lds_bytes(NF);
+  WaveState w;
+  w.lists = reinterpret_cast<uint2*>(my);
+  w.wstats = reinterpret_cast<unsigned long long*>(my + NF * kList * 8);
+  w.pws = reinterpret_cast<uint32_t*>(my + NF * kList * 8 + kMaxFam * 8);
+  w.heads = w.pws + NF * 64;
+  w.tails = w.heads + kMaxFam;
+  if (lane < kMaxFam) w.wstats[lane] = 0;
   wave_sync();
   for (;;) {
     uint32_t item = 0;
     if (lane == 0) item = atomicAdd(work_counter, 1u);
     item = __builtin_amdgcn_readlane(item, 0);
     if (item >= a.n_items) break;
-    const uint32_t slot = item / a.chunks_per_slot;
+    uint32_t slot, run;
+    if (a.interleave) {
+      slot = item % a.n_slots;
+      run = item / a.n_slots;
+    } else {
+      slot = item / a.chunks_per_slot;
+      run = item - slot * a.chunks_per_slot;
+    }
     const uint32_t s = slot ? a.slot_strand1 : a.slot_strand0;
-    const uint32_t cs = a.valid_begin + (item - slot * a.chunks_per_slot) * a.chunk;
+    const uint32_t cs = a.valid_begin + run * a.chunk;
     const uint32_t ce = min(cs + a.chunk, a.valid_end);
-    scan_run<NF>(a, t, s, cs, ce, lists, wstats, lane);
+    scan_run<NF>(a, t, w, s, cs, ce, lane);
   }
   wave_sync();
-  if (lane < NF && wstats[lane]) atomicAdd(&fam_windows[lane], wstats[lane]);
+  if (lane < NF && w.wstats[lane]) atomicAdd(&fam_windows[lane], w.wstats[lane]);
 }
 
 using KernelFn = void (*)(Args, const Fam*, const uint32_t*, const int32_t*, const uint32_t*,
@@ -332,6 +425,11 @@ struct bgo_plan {
   int device = 0;
   int32_t n_guides = 0;
   int64_t hit_cap = 0;
+  double score[21] = {};
+  size_t mem_bytes = 0;               // size of d_mem
+  // host mirrors of the device tables (alive until the next upload completes)
+  std::vector<uint32_t> h_g01, h_gaux, h_gid, h_gnm;
+  std::vector<int32_t> h_gk;
   bgo_tuning tune{};
   // families, grouped into launches of <= kMaxFam
   std::vector<Fam> fams;              // all families (g ranges into the padded table)
@@ -353,6 +451,8 @@ struct bgo_plan {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_mask = -1;
+  int n_cu = 0;
+  int resolved_blocks_per_cu = 0;
   std::vector<int> group_blocks;     // persistent blocks per launch group
   std::vector<size_t> group_lds;     // dynamic LDS bytes per launch group
   int threads = 0;
@@ -454,21 +554,8 @@ uint64_t fam_key(const bgo_guide_desc& d) {
 
 Fam make_fam(const bgo_guide_desc& d) {
   Fam F{};
-  uint32_t a = 0, c = 0, gg = 0, t = 0, pam_bits = 0;
-  for (int i = 0; i < d.pam_len; ++i) {
-    const uint32_t nib = d.pam_mask[i] & 0xFu;
-    const uint32_t bit = 1u << (20 + i);
-    pam_bits |= bit;
-    if (nib & 1u) a |= bit;
-    if (nib & 2u) c |= bit;
-    if (nib & 4u) gg |= bit;
-    if (nib & 8u) t |= bit;
-  }
-  const uint32_t rest = ~pam_bits;
-  F.mA = a | rest;
-  F.mC = c | rest;
-  F.mG = gg | rest;
-  F.mT = t | rest;
+  for (int i = 0; i < d.pam_len; ++i) F.nibs |= (d.pam_mask[i] & 0xFu) << (4 * i);
+  F.plen = d.pam_len;
   return F;
 }
 
@@ -492,22 +579,24 @@ int plan_configure_launch(bgo_plan* P) {
   int n_cu = 0;
   BGO_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, P->device));
   P->threads = P->tune.waves_per_block * 64;
+  P->group_blocks.clear();
+  P->group_lds.clear();
   int min_occ = 1 << 30;
   for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
     const int nf = std::min<int>(bk::kMaxFam, static_cast<int>(P->fams.size()) - P->group_first[gi]);
-    const size_t per_wave = static_cast<size_t>(nf) * bk::kList * sizeof(uint2) +
-                            bk::kMaxFam * sizeof(unsigned long long);
+    const size_t per_wave = bk::wave_lds_bytes(nf);
     const size_t lds = per_wave * P->tune.waves_per_block;
     int occ = 0;
     BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bk::valu_kernel_for(nf), P->threads,
                                                          lds));
     if (occ <= 0) occ = 1;
     if (P->tune.blocks_per_cu > 0) occ = std::min(occ, P->tune.blocks_per_cu);
+    P->n_cu = n_cu;
     min_occ = std::min(min_occ, occ);
     P->group_blocks.push_back(occ * n_cu);
     P->group_lds.push_back(lds);
   }
-  P->tune.blocks_per_cu = P->group_first.empty() ? 0 : min_occ;
+  if (!P->group_first.empty()) P->resolved_blocks_per_cu = min_occ;
   return BGO_OK;
 }
 
@@ -525,12 +614,10 @@ extern "C" int bgo_plan_destroy(bgo_plan* P) {
   return BGO_OK;
 }
 
-extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int32_t n_guides,
-                               const double* score_table, const bgo_tuning* tuning,
-                               int64_t hit_capacity, bgo_plan** out) {
-  if (!G || !out || n_guides < 0 || (n_guides > 0 && !guides) || !score_table ||
-      hit_capacity < 0)
-    return fail(BGO_ERR_ARG, "bgo_plan_create: bad arguments");
+namespace {
+
+int validate_guides(const bgo_guide_desc* guides, int32_t n_guides) {
+  if (n_guides < 0 || (n_guides > 0 && !guides)) return fail(BGO_ERR_ARG, "bad guide array");
   for (int32_t i = 0; i < n_guides; ++i) {
     const auto& d = guides[i];
     if (d.pam_len < 1 || d.pam_len > BGO_MAX_PAM_LEN)
@@ -538,20 +625,14 @@ extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int3
     if (d.max_mm > 20) return fail(BGO_ERR_ARG, "guide %d: max_mm %d > 20", i, d.max_mm);
     if (d.editor > 1) return fail(BGO_ERR_ARG, "guide %d: editor %d", i, d.editor);
   }
-  BGO_TRY(injected());
-  auto* P = new bgo_plan();
-  auto bail = [&](int rc) {
-    bgo_plan_destroy(P);
-    return rc;
-  };
-  P->g = G;
-  P->device = G->device;
-  P->n_guides = n_guides;
-  P->hit_cap = hit_capacity;
-  if (hipSetDevice(G->device) != hipSuccess) return bail(fail(BGO_ERR_HIP, "hipSetDevice"));
-  if (int rc = resolve_tuning(P, tuning); rc != BGO_OK) return bail(rc);
+  return BGO_OK;
+}
 
-  // Group guides by PAM family, in order of first appearance.
+// Group guides by PAM family (order of first appearance), build the padded
+// tables, grow the device allocation if needed and upload on the plan stream.
+// The previous launch must be complete (callers synchronise first).
+int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides) {
+  BGO_TRY(validate_guides(guides, n_guides));
   std::map<uint64_t, int> fam_index;
   std::vector<std::vector<int32_t>> members;
   std::vector<Fam> proto;
@@ -559,16 +640,26 @@ extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int3
     const uint64_t key = fam_key(guides[i]);
     auto it = fam_index.find(key);
     if (it == fam_index.end()) {
-      fam_index.emplace(key, static_cast<int>(members.size()));
+      it = fam_index.emplace(key, static_cast<int>(members.size())).first;
       members.emplace_back();
       proto.push_back(make_fam(guides[i]));
-      it = fam_index.find(key);
     }
     members[it->second].push_back(i);
   }
   const int n_fam = static_cast<int>(members.size());
-  std::vector<uint32_t> g01, gaux, gid, gnm;
-  std::vector<int32_t> gk;
+  P->fams.clear();
+  P->fam_guides.clear();
+  P->group_first.clear();
+  P->h_g01.clear();
+  P->h_gaux.clear();
+  P->h_gid.clear();
+  P->h_gnm.clear();
+  P->h_gk.clear();
+  auto& g01 = P->h_g01;
+  auto& gaux = P->h_gaux;
+  auto& gid = P->h_gid;
+  auto& gnm = P->h_gnm;
+  auto& gk = P->h_gk;
   for (int f = 0; f < n_fam; ++f) {
     Fam F = proto[f];
     F.g_begin = static_cast<uint32_t>(gk.size());
@@ -608,8 +699,8 @@ extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int3
   }
   for (int f = 0; f < n_fam; f += bk::kMaxFam) P->group_first.push_back(f);
   const size_t n_pad = gk.size();
+  P->n_guides = n_guides;
 
-  // One device allocation for the tables and counters.
   P->n_counter_words = 1 + static_cast<uint32_t>(P->group_first.size()) + n_fam;
   size_t off = 0;
   const size_t o_fam = off; off = align256(off + std::max<size_t>(n_fam, 1) * sizeof(Fam));
@@ -620,7 +711,14 @@ extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int3
   const size_t o_gnm = off; off = align256(off + std::max<size_t>(n_pad, 1) * 8);
   const size_t o_score = off; off = align256(off + 21 * sizeof(double));
   const size_t o_cnt = off; off = align256(off + P->n_counter_words * 8);
-  if (hipMalloc(&P->d_mem, off) != hipSuccess) return bail(fail(BGO_ERR_HIP, "hipMalloc plan"));
+  if (off > P->mem_bytes) {
+    const size_t want = std::max(off, P->mem_bytes * 3 / 2);
+    if (P->d_mem) (void)hipFree(P->d_mem);
+    P->d_mem = nullptr;
+    P->mem_bytes = 0;
+    BGO_HIP(hipMalloc(&P->d_mem, want));
+    P->mem_bytes = want;
+  }
   char* b = static_cast<char*>(P->d_mem);
   P->d_fam = reinterpret_cast<Fam*>(b + o_fam);
   P->d_g01 = reinterpret_cast<uint32_t*>(b + o_g01);
@@ -630,25 +728,83 @@ extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int3
   P->d_gnm = reinterpret_cast<uint32_t*>(b + o_gnm);
   P->d_score = reinterpret_cast<double*>(b + o_score);
   P->d_counters = reinterpret_cast<unsigned long long*>(b + o_cnt);
-  bool ok = true;
-  if (n_fam) ok &= hipMemcpy(P->d_fam, P->fams.data(), n_fam * sizeof(Fam), hipMemcpyHostToDevice) == hipSuccess;
+  hipStream_t st = P->stream;
+  if (n_fam) BGO_HIP(hipMemcpyAsync(P->d_fam, P->fams.data(), n_fam * sizeof(Fam), hipMemcpyHostToDevice, st));
   if (n_pad) {
-    ok &= hipMemcpy(P->d_g01, g01.data(), n_pad * 8, hipMemcpyHostToDevice) == hipSuccess;
-    ok &= hipMemcpy(P->d_gk, gk.data(), n_pad * 4, hipMemcpyHostToDevice) == hipSuccess;
-    ok &= hipMemcpy(P->d_gaux, gaux.data(), n_pad * 4, hipMemcpyHostToDevice) == hipSuccess;
-    ok &= hipMemcpy(P->d_gid, gid.data(), n_pad * 4, hipMemcpyHostToDevice) == hipSuccess;
-    ok &= hipMemcpy(P->d_gnm, gnm.data(), n_pad * 8, hipMemcpyHostToDevice) == hipSuccess;
+    BGO_HIP(hipMemcpyAsync(P->d_g01, g01.data(), n_pad * 8, hipMemcpyHostToDevice, st));
+    BGO_HIP(hipMemcpyAsync(P->d_gk, gk.data(), n_pad * 4, hipMemcpyHostToDevice, st));
+    BGO_HIP(hipMemcpyAsync(P->d_gaux, gaux.data(), n_pad * 4, hipMemcpyHostToDevice, st));
+    BGO_HIP(hipMemcpyAsync(P->d_gid, gid.data(), n_pad * 4, hipMemcpyHostToDevice, st));
+    BGO_HIP(hipMemcpyAsync(P->d_gnm, gnm.data(), n_pad * 8, hipMemcpyHostToDevice, st));
   }
-  ok &= hipMemcpy(P->d_score, score_table, 21 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
-  ok &= hipMemset(P->d_counters, 0, P->n_counter_words * 8) == hipSuccess;
-  if (!ok) return bail(fail(BGO_ERR_HIP, "upload plan tables"));
-  if (hit_capacity > 0 && hipMalloc(&P->d_hits, hit_capacity * sizeof(bgo_hit)) != hipSuccess)
-    return bail(fail(BGO_ERR_HIP, "hipMalloc hit buffer (%lld records)", (long long)hit_capacity));
+  BGO_HIP(hipMemcpyAsync(P->d_score, P->score, 21 * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(bk::reset_counters_kernel, dim3(1), dim3(256), 0, st, P->d_counters,
+                     P->n_counter_words);
+  BGO_HIP(hipGetLastError());
+  if (P->graph) {  // kernel arguments changed
+    (void)hipGraphExecDestroy(P->graph);
+    P->graph = nullptr;
+    P->graph_mask = -1;
+  }
+  return plan_configure_launch(P);
+}
+
+}  // namespace
+
+extern "C" int bgo_plan_create(bgo_genome* G, const bgo_guide_desc* guides, int32_t n_guides,
+                               const double* score_table, const bgo_tuning* tuning,
+                               int64_t hit_capacity, bgo_plan** out) {
+  if (!G || !out || !score_table || hit_capacity < 0)
+    return fail(BGO_ERR_ARG, "bgo_plan_create: bad arguments");
+  BGO_TRY(validate_guides(guides, n_guides));
+  BGO_TRY(injected());
+  auto* P = new bgo_plan();
+  auto bail = [&](int rc) {
+    bgo_plan_destroy(P);
+    return rc;
+  };
+  P->g = G;
+  P->device = G->device;
+  P->hit_cap = hit_capacity;
+  std::memcpy(P->score, score_table, sizeof(P->score));
+  if (hipSetDevice(G->device) != hipSuccess) return bail(fail(BGO_ERR_HIP, "hipSetDevice"));
+  if (int rc = resolve_tuning(P, tuning); rc != BGO_OK) return bail(rc);
   if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&P->ev0) != hipSuccess || hipEventCreate(&P->ev1) != hipSuccess)
     return bail(fail(BGO_ERR_HIP, "stream/event create"));
-  if (int rc = plan_configure_launch(P); rc != BGO_OK) return bail(rc);
+  if (hit_capacity > 0 && hipMalloc(&P->d_hits, hit_capacity * sizeof(bgo_hit)) != hipSuccess)
+    return bail(fail(BGO_ERR_HIP, "hipMalloc hit buffer (%lld records)", (long long)hit_capacity));
+  if (int rc = plan_set_guides(P, guides, n_guides); rc != BGO_OK) return bail(rc);
+  if (hipStreamSynchronize(P->stream) != hipSuccess) return bail(fail(BGO_ERR_HIP, "upload plan tables"));
   *out = P;
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides) {
+  if (!P) return fail(BGO_ERR_ARG, "null plan");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  return plan_set_guides(P, guides, n_guides);
+}
+
+extern "C" int bgo_plan_set_hit_capacity(bgo_plan* P, int64_t hit_capacity) {
+  if (!P || hit_capacity < 0) return fail(BGO_ERR_ARG, "bad arguments");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  if (P->d_hits) (void)hipFree(P->d_hits);
+  P->d_hits = nullptr;
+  P->hit_cap = 0;
+  if (hit_capacity > 0) BGO_HIP(hipMalloc(&P->d_hits, hit_capacity * sizeof(bgo_hit)));
+  P->hit_cap = hit_capacity;
+  if (P->graph) {
+    (void)hipGraphExecDestroy(P->graph);
+    P->graph = nullptr;
+    P->graph_mask = -1;
+  }
   return BGO_OK;
 }
 
@@ -686,6 +842,8 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
     a.chunk = chunk;
     a.chunks_per_slot = static_cast<uint32_t>(cps);
     a.n_items = static_cast<uint32_t>(cps * n_slots);
+    a.n_slots = n_slots;
+    a.interleave = P->tune.reserved[0] ? 1u : 0u;
     a.n_fam = static_cast<uint32_t>(nf);
     a.hit_cap = static_cast<unsigned long long>(P->hit_cap);
     hipLaunchKernelGGL(bk::valu_kernel_for(nf), dim3(P->group_blocks[gi]), dim3(P->threads),
@@ -749,7 +907,8 @@ extern "C" int bgo_plan_wait(bgo_plan* P, int64_t* n_hits_out, float* kernel_ms_
   BGO_HIP(hipSetDevice(P->device));
   BGO_HIP(hipStreamSynchronize(P->stream));
   unsigned long long n = 0;
-  BGO_HIP(hipMemcpy(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost));
+  BGO_HIP(hipMemcpyAsync(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost, P->stream));
+  BGO_HIP(hipStreamSynchronize(P->stream));
   *n_hits_out = static_cast<int64_t>(n);
   if (kernel_ms_out) {
     float ms = 0.f;
@@ -764,10 +923,15 @@ extern "C" int bgo_plan_fetch(bgo_plan* P, bgo_hit* out, int64_t cap, int64_t* n
   BGO_TRY(injected());
   std::lock_guard<std::mutex> lock(P->mu);
   BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
   unsigned long long n = 0;
-  BGO_HIP(hipMemcpy(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost));
+  BGO_HIP(hipMemcpyAsync(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost, P->stream));
+  BGO_HIP(hipStreamSynchronize(P->stream));
   const int64_t m = std::min<int64_t>({static_cast<int64_t>(n), P->hit_cap, cap});
-  if (m > 0) BGO_HIP(hipMemcpy(out, P->d_hits, m * sizeof(bgo_hit), hipMemcpyDeviceToHost));
+  if (m > 0) {
+    BGO_HIP(hipMemcpyAsync(out, P->d_hits, m * sizeof(bgo_hit), hipMemcpyDeviceToHost, P->stream));
+    BGO_HIP(hipStreamSynchronize(P->stream));
+  }
   *n_copied = std::max<int64_t>(m, 0);
   return BGO_OK;
 }
@@ -775,6 +939,7 @@ extern "C" int bgo_plan_fetch(bgo_plan* P, bgo_hit* out, int64_t cap, int64_t* n
 extern "C" int bgo_plan_get_tuning(const bgo_plan* P, bgo_tuning* out) {
   if (!P || !out) return fail(BGO_ERR_ARG, "null pointer");
   *out = P->tune;
+  if (out->blocks_per_cu <= 0) out->blocks_per_cu = P->resolved_blocks_per_cu;
   return BGO_OK;
 }
 
@@ -788,8 +953,10 @@ extern "C" int bgo_plan_family_stats(const bgo_plan* P, int32_t cap, int32_t* n_
   std::vector<unsigned long long> w(std::max(n_fam, 1));
   if (n_fam) {
     BGO_HIP(hipSetDevice(P->device));
-    BGO_HIP(hipMemcpy(w.data(), P->d_counters + 1 + P->group_first.size(),
-                      n_fam * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    BGO_HIP(hipStreamSynchronize(P->stream));
+    BGO_HIP(hipMemcpyAsync(w.data(), P->d_counters + 1 + P->group_first.size(),
+                           n_fam * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->stream));
+    BGO_HIP(hipStreamSynchronize(P->stream));
   }
   for (int f = 0; f < std::min(cap, n_fam); ++f) {
     if (windows_out) windows_out[f] = static_cast<int64_t>(w[f]);

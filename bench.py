@@ -89,6 +89,49 @@ def cpu_baseline(workload, max_mm: int, target_s: float = 12.0) -> dict:
                       f"({dt:.1f} s, {threads} threads)"}
 
 
+def serving_leg(dg, pool, max_mm: int, rates, duration: float, seed: int = 0) -> dict:
+    """Poisson arrivals of guide-set requests (log-uniform size 1-500 drawn
+    from `pool`) at each rate for `duration` seconds; p50/p99 latency from
+    arrival to complete site records, and sustained guide*Gbp/s."""
+    from beguidedoff_amd.batched import GuideSpec
+    from beguidedoff_amd.serving import ScanServer, latency_stats
+
+    rng = np.random.default_rng(seed)
+    gbp = dg.total_bp / 1e9
+    out = {}
+    for rate in rates:
+        with ScanServer(dg, n_segments=16, hit_budget=1 << 24, pipeline=2) as srv:
+            t0 = time.perf_counter()
+            handles = []
+            t_next = t0
+            while True:
+                t_next += rng.exponential(1.0 / rate)
+                if t_next - t0 > duration:
+                    break
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                n = int(np.exp(rng.uniform(0, np.log(500))))
+                idx = rng.choice(len(pool), size=n, replace=False)
+                specs = [GuideSpec(pool[j].guide_seq, pool[j].pam, max_mm, pool[j].editor_type,
+                                   (pool[j].edit_window_start, pool[j].edit_window_end))
+                         for j in idx]
+                handles.append(srv.submit(specs))
+            for h in handles:
+                h.result(timeout=600)
+            wall = time.perf_counter() - t0
+            st = latency_stats(handles)
+            guides = sum(len(h.specs) for h in handles)
+            st.update({"rate_req_s": rate, "requests": len(handles), "guides": guides,
+                       "duration_s": round(wall, 2), "offered_s": duration,
+                       "guide_gbp_s": round(guides * gbp / wall, 1),
+                       "launches": srv.launches, "reruns": srv.reruns,
+                       "gpu_busy_frac": round(srv.busy_s / wall, 3)})
+            out[str(rate)] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}
+            log(f"serving {rate} req/s: {out[str(rate)]}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,8 +143,12 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tuning", default="", help="JSON dict of bgo_tuning fields")
+    ap.add_argument("--tuning", default="auto",
+                    help="'auto' (autotune on a genome slice), '' (library default) or a JSON dict")
     ap.add_argument("--eager", action="store_true", help="launch without the hipGraph")
+    ap.add_argument("--rates", default="2,8,32", help="serving arrival rates (req/s); '' skips")
+    ap.add_argument("--serving-duration", type=float, default=15.0,
+                    help="seconds of Poisson arrivals per rate")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -130,7 +177,16 @@ def main() -> int:
     t0 = time.perf_counter()
     dg = DeviceGenome(g, g["n_mask_le_fwd"], g["n_mask_le_rc"])
     log(f"genome resident in HBM ({dg.device_bytes() / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s")
-    tuning = json.loads(args.tuning) if args.tuning else None
+    tune_trials = None
+    if args.tuning == "auto":
+        from beguidedoff_amd.autotune import autotune
+
+        t0 = time.perf_counter()
+        tuning, trials = autotune(dg, specs, use_cache=False)
+        tune_trials = [{"tuning": t, "sample_ms": round(ms, 3)} for t, ms in trials[:4]]
+        log(f"autotune ({len(trials)} candidates, {time.perf_counter() - t0:.1f} s): {tuning}")
+    else:
+        tuning = json.loads(args.tuning) if args.tuning else None
     plan = Plan(dg, specs, tuning=tuning, hit_capacity=default_capacity(len(specs)))
 
     def step():
@@ -175,6 +231,7 @@ def main() -> int:
                    "pams": sorted({s.pam for s in specs}), "engine": "batched VALU",
                    "graph": not args.eager, "tuning": plan.tuning()},
         "sites": int(n),
+        "autotune": tune_trials,
         "kernel_ms": round(kms, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
                      "unit": "Tops/s", "frac": round(achieved / peak, 4), "traffic": None,
@@ -213,6 +270,10 @@ def main() -> int:
                         "steps": args.plain_steps}
         out["speedup_vs_plain"] = round(pk / kms, 2)
         out["sites_equal_plain"] = bool(plain_sites == n)
+
+    if args.rates:
+        rates = [float(r) for r in args.rates.split(",") if r]
+        out["serving"] = serving_leg(dg, w.guides, args.max_mm, rates, args.serving_duration)
 
     if not args.no_cpu:
         t0 = time.perf_counter()

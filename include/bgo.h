@@ -161,8 +161,8 @@ typedef struct bgo_tuning {
   int32_t waves_per_block;   /* 1..16 */
   int32_t blocks_per_cu;     /* resident blocks requested per CU */
   int32_t engine;            /* 0 auto, 1 VALU xor/popcount, 2 int8 MFMA */
-  int32_t guide_unroll;      /* 4 or 8 (VALU engine) */
-  int32_t reserved[3];
+  int32_t guide_unroll;      /* guides per compare step (fixed at 8) */
+  int32_t reserved[3];       /* [0]: 1 = work items alternate strands */
 } bgo_tuning;
 
 typedef struct bgo_plan bgo_plan;
@@ -177,6 +177,16 @@ int bgo_plan_create(bgo_genome* g, const bgo_guide_desc* guides,
                     const bgo_tuning* tuning, int64_t hit_capacity,
                     bgo_plan** out);
 int bgo_plan_destroy(bgo_plan* p);
+
+/* Replace the guide set of a plan (any size; device tables grow as needed).
+ * Waits for the plan's previous launch, then uploads on the plan's stream.
+ * Used by the serving loop to admit and retire requests between launches. */
+int bgo_plan_set_guides(bgo_plan* p, const bgo_guide_desc* guides,
+                        int32_t n_guides);
+
+/* Replace the hit buffer with one of hit_capacity records (waits for the
+ * previous launch; the old records are discarded). */
+int bgo_plan_set_hit_capacity(bgo_plan* p, int64_t hit_capacity);
 
 /* Enqueue one scan of positions [pos_begin, pos_end) (clamped to
  * [0, total_bp - 26), cuda_search.py:211-213) on the strands of strand_mask

@@ -205,3 +205,25 @@ def test_graph_replays_match_eager(gpu):
         plan.wait()
     plan.close()
     assert eager == oracle_set(genome, guides, max_mm)
+
+
+def test_tunings_give_identical_sites(gpu):
+    """Tile length, waves per block, occupancy and strand interleave change
+    only the schedule, never the sites."""
+    contigs, guides, max_mm = build_case(12, length=(60_000, 90_000), n_guides=10)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, max_mm) for s, p in guides]
+    want = oracle_set(genome, guides, max_mm)
+    for t in ({"chunk_positions": 64}, {"chunk_positions": 4096, "interleave": 1},
+              {"chunk_positions": 1 << 20, "waves_per_block": 1, "blocks_per_cu": 1},
+              {"waves_per_block": 2, "blocks_per_cu": 3, "interleave": 1}):
+        r = scan(dg, specs, tuning=t)
+        assert hit_set(r.hits) == want, t
+    from beguidedoff_amd.autotune import autotune
+    import os
+    os.environ["BGO_TUNE_CACHE"] = "/tmp/bgo_tune_test.json"
+    best, trials = autotune(dg, specs, sample_positions=100_000, use_cache=False,
+                            space={"chunk_positions": [4096, 65536], "waves_per_block": [4],
+                                   "blocks_per_cu": [0], "interleave": [0, 1]})
+    assert len(trials) == 4 and hit_set(scan(dg, specs, tuning=best).hits) == want
