@@ -1,7 +1,8 @@
 """Autotuner of the batched kernel's launch shape.
 
-Tunables (bgo_tuning): positions per work item (genome tile length), waves
-per block, resident blocks per CU, and strand interleave of the work items.
+Tunables (bgo_tuning): compare engine (VALU xor/popcount or int8 MFMA),
+positions per work item (genome tile length), waves per block, resident
+blocks per CU, and strand interleave of the work items.
 Each candidate runs the real kernel on a slice of the resident genome with
 the caller's guide set (both strands) and is timed with HIP events; the
 fastest is kept.  Results are cached per (device, PAM-family mix, guide
@@ -20,7 +21,8 @@ from . import _lib
 from .batched import DeviceGenome, GuideSpec, Plan
 
 DEFAULT_SPACE = {
-    "chunk_positions": [32768, 65536, 131072, 262144],
+    "engine": [1, 2],
+    "chunk_positions": [32768, 131072],
     "waves_per_block": [2, 4],
     "blocks_per_cu": [0],
     "interleave": [0, 1],

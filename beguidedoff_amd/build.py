@@ -35,6 +35,8 @@ HIPCC_FLAGS = [
     "-fPIC",
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in VGPRs (not AGPRs): the epilogue reads them with plain VALU ops
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

@@ -85,6 +85,7 @@ struct Tab {
   const double* __restrict__ score_tab;  // [21]
   bgo_hit* __restrict__ hits;
   unsigned long long* __restrict__ hit_count;
+  const int4* __restrict__ btab;         // MFMA engine: [guide tile][j][lane] int8 x16
 };
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
@@ -224,6 +225,101 @@ __device__ inline uint32_t allowed(uint32_t nib, uint32_t x0, uint32_t x1) {
 // per PAM position for 32 positions), then the passing windows are appended
 // to the family's ring buffer in LDS; every 64 buffered windows are compared
 // against the family's guides (one window per lane).
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// 16 bits -> 16 int8 of +1 (bit 0) / -1 (bit 1), 4 per dword.
+__device__ inline v4i expand16(uint32_t bits) {
+  v4i r;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t spread = (((bits >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u;
+    r[q] = static_cast<int>(__builtin_amdgcn_perm(0u, 0x0000FF01u, spread));
+  }
+  return r;
+}
+
+// int8 MFMA engine of compare_list.  Each window and guide is a 64-element
+// int8 vector: for every base the tetrahedral code (+-1 per plane of b1, b0,
+// b0^b1; elements 60..63 are 0 in the guide operand), so a window-guide dot
+// product is 60 - 4 * (2-bit mismatches).  One wave multiplies up to 64
+// windows (two 32-row A tiles, one window per lane pair) by 32-guide B tiles
+// with v_mfma_i32_32x32x32_i8 (K = 2 x 32).  Accumulators start at
+// 4*kmax - 60, so a non-negative entry marks a candidate with <= kmax raw
+// mismatches; the common "none" case is one AND-reduction of sign bits.
+// Lane layouts (checked on gfx950): A/B row or column = lane % 32, lane group
+// lane / 32 selects 16 of the 32 k of each instruction; D register r of lane
+// l is row 8*(r/4) + 4*(l/32) + r%4, column l % 32.
+__device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam& F, uint32_t s,
+                                         const uint2* L, uint32_t head, uint32_t n, uint32_t cs,
+                                         uint32_t lane) {
+  const uint32_t m = lane & 31u, g = lane >> 5;
+  v4i A[2][2];
+#pragma unroll
+  for (int tile = 0; tile < 2; ++tile) {
+    const uint32_t widx = 32u * tile + m;
+    uint2 e = make_uint2(0u, 0u);
+    if (widx < n) e = L[(head + widx) & (kList - 1)];
+    const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
+    // 64-bit string: W1 at bits 0..19, W0 at 20..39, W0^W1 at 40..59
+    const uint32_t lo = W1 | (W0 << 20);
+    const uint32_t hi = (W0 >> 12) | ((W0 ^ W1) << 8);
+    A[tile][0] = expand16((lo >> (16 * g)) & 0xFFFFu);
+    A[tile][1] = expand16((hi >> (16 * g)) & 0xFFFFu);
+  }
+  const int bias = 4 * static_cast<int>(F.kmax) - 60;
+  v16i cinit;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) cinit[r] = bias;
+  const bool two = n > 32;
+  for (uint32_t gt = F.g_begin; gt < F.g_end; gt += 32) {
+    const int4* __restrict__ bp = t.btab + (gt / 32) * 128;
+    const int4 b0 = bp[lane], b1 = bp[64 + lane];
+    const v4i B0 = {b0.x, b0.y, b0.z, b0.w}, B1 = {b1.x, b1.y, b1.z, b1.w};
+    v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B0, cinit, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][1], B1, acc0, 0, 0, 0);
+    v16i acc1 = cinit;
+    if (two) {
+      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, cinit, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
+    }
+    uint32_t all = static_cast<uint32_t>(acc0[0]);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) all &= static_cast<uint32_t>(acc0[r]);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) all &= static_cast<uint32_t>(acc1[r]);
+    if (!ballot(static_cast<int>(all) >= 0)) continue;
+    // Rare: entries with a non-negative accumulator -> exact per-pair check.
+    uint32_t bits = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      bits |= static_cast<uint32_t>(acc0[r] >= 0) << r;
+      bits |= static_cast<uint32_t>(acc1[r] >= 0) << (16 + r);
+    }
+    while (ballot(bits != 0)) {
+      bool cand = false;
+      uint32_t gi = 0, pos = 0, W0 = 0, W1 = 0, c = 0;
+      if (bits) {
+        const uint32_t bit = __builtin_ctz(bits);
+        bits &= bits - 1;
+        const uint32_t r = bit & 15u, tile = bit >> 4;
+        const uint32_t row = 8u * (r >> 2) + 4u * g + (r & 3u);
+        const uint32_t widx = 32u * tile + row;
+        gi = gt + m;
+        if (widx < n) {
+          const uint2 e = L[(head + widx) & (kList - 1)];
+          W0 = e.x & kWin;
+          W1 = e.y & kWin;
+          pos = cs + ((e.x >> 20) | ((e.y >> 20) << 12));
+          c = __popc((W0 ^ t.g01[2 * gi]) | (W1 ^ t.g01[2 * gi + 1]));
+          cand = static_cast<int>(c) <= t.gk[gi];
+        }
+      }
+      if (ballot(cand)) record_candidates(a, t, s, gi, pos, W0, W1, c, cand, lane);
+    }
+  }
+}
+
 // Per-wave LDS state of the scan loop.
 struct WaveState {
   uint2* lists;                  // [NF][kList] ring buffers of windows
@@ -235,6 +331,7 @@ struct WaveState {
 
 // Append this lane's passing windows (bits of pw in block bb) to family f's
 // ring buffer, comparing every 64 buffered windows.  All lanes call this.
+template <int ENGINE>
 __device__ inline void append_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
                                      uint32_t s, uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
                                      uint32_t cs, uint32_t lane) {
@@ -258,7 +355,10 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
     }
     wave_sync();
     if (tail - head < 64) break;
-    compare_list(a, t, t.fam[f], s, L, head, 64, cs, lane);
+    if (ENGINE == 2)
+      compare_list_mfma(a, t, t.fam[f], s, L, head, 64, cs, lane);
+    else
+      compare_list(a, t, t.fam[f], s, L, head, 64, cs, lane);
     head += 64;
     if (lane == 0) w.wstats[f] += 64;
     wave_sync();
@@ -272,7 +372,7 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
 // per PAM position for 32 positions), then the passing windows are appended
 // to the family's ring buffer in LDS; every 64 buffered windows are compared
 // against the family's guides (one window per lane).
-template <int NF>
+template <int NF, int ENGINE>
 __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w, uint32_t s,
                                 uint32_t cs, uint32_t ce, uint32_t lane) {
   const uint2* __restrict__ planes = s ? a.planes1 : a.planes0;
@@ -321,7 +421,7 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
 #pragma nounroll
     for (uint32_t f = 0; f < NF; ++f) {
       const uint32_t pw = w.pws[f * 64 + lane];
-      if (ballot(pw != 0)) append_family(a, t, w, f, s, pw, bb, qa, qb, cs, lane);
+      if (ballot(pw != 0)) append_family<ENGINE>(a, t, w, f, s, pw, bb, qa, qb, cs, lane);
     }
   }
 #pragma nounroll
@@ -329,22 +429,25 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
     const uint32_t head = uniform(w.heads[f]);
     const uint32_t c = uniform(w.tails[f]) - head;
     if (c) {
-      compare_list(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
+      if (ENGINE == 2)
+        compare_list_mfma(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
+      else
+        compare_list(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
       if (lane == 0) w.wstats[f] += c;
     }
   }
   wave_sync();
 }
 
-template <int NF>
+template <int NF, int ENGINE>
 __global__ __launch_bounds__(kMaxBlock) void batched_valu_kernel(
     Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
     const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
     const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gnm,
     const double* __restrict__ score_tab, bgo_hit* __restrict__ hits,
     unsigned long long* __restrict__ hit_count, unsigned int* __restrict__ work_counter,
-    unsigned long long* __restrict__ fam_windows) {
-  const Tab t{fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count};
+    unsigned long long* __restrict__ fam_windows, const int4* __restrict__ btab) {
+  const Tab t{fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count, btab};
   extern __shared__ uint2 lds[];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
@@ -376,7 +479,7 @@ __global__ __launch_bounds__(kMaxBlock) void batched_valu_kernel(
     const uint32_t s = slot ? a.slot_strand1 : a.slot_strand0;
     const uint32_t cs = a.valid_begin + run * a.chunk;
     const uint32_t ce = min(cs + a.chunk, a.valid_end);
-    scan_run<NF>(a, t, w, s, cs, ce, lane);
+    scan_run<NF, ENGINE>(a, t, w, s, cs, ce, lane);
   }
   wave_sync();
   if (lane < NF && w.wstats[lane]) atomicAdd(&fam_windows[lane], w.wstats[lane]);
@@ -384,19 +487,24 @@ __global__ __launch_bounds__(kMaxBlock) void batched_valu_kernel(
 
 using KernelFn = void (*)(Args, const Fam*, const uint32_t*, const int32_t*, const uint32_t*,
                          const uint32_t*, const uint32_t*, const double*, bgo_hit*,
-                         unsigned long long*, unsigned int*, unsigned long long*);
+                         unsigned long long*, unsigned int*, unsigned long long*, const int4*);
 
-inline KernelFn valu_kernel_for(int nf) {
+template <int ENGINE>
+inline KernelFn kernel_for_nf(int nf) {
   switch (nf) {
-    case 1: return batched_valu_kernel<1>;
-    case 2: return batched_valu_kernel<2>;
-    case 3: return batched_valu_kernel<3>;
-    case 4: return batched_valu_kernel<4>;
-    case 5: return batched_valu_kernel<5>;
-    case 6: return batched_valu_kernel<6>;
-    case 7: return batched_valu_kernel<7>;
-    default: return batched_valu_kernel<8>;
+    case 1: return batched_valu_kernel<1, ENGINE>;
+    case 2: return batched_valu_kernel<2, ENGINE>;
+    case 3: return batched_valu_kernel<3, ENGINE>;
+    case 4: return batched_valu_kernel<4, ENGINE>;
+    case 5: return batched_valu_kernel<5, ENGINE>;
+    case 6: return batched_valu_kernel<6, ENGINE>;
+    case 7: return batched_valu_kernel<7, ENGINE>;
+    default: return batched_valu_kernel<8, ENGINE>;
   }
+}
+
+inline KernelFn kernel_for(int nf, int engine) {
+  return engine == 2 ? kernel_for_nf<2>(nf) : kernel_for_nf<1>(nf);
 }
 
 __global__ void reset_counters_kernel(unsigned long long* c, uint32_t n) {
@@ -445,6 +553,8 @@ struct bgo_plan {
   uint32_t* d_gnm = nullptr;
   double* d_score = nullptr;
   unsigned long long* d_counters = nullptr;  // [1 hit count][groups work counters][fam windows]
+  int8_t* d_btab = nullptr;                  // MFMA guide operand tiles (engine 2)
+  std::vector<int8_t> h_btab;
   uint32_t n_counter_words = 0;
   bgo_hit* d_hits = nullptr;
   hipStream_t stream = nullptr;
@@ -569,8 +679,8 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
     return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");
   if (r.engine <= 0) r.engine = 1;
-  if (r.engine != 1) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
-  r.guide_unroll = bk::kUnroll;
+  if (r.engine != 1 && r.engine != 2) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
+  r.guide_unroll = r.engine == 2 ? 32 : bk::kUnroll;
   P->tune = r;
   return BGO_OK;
 }
@@ -587,7 +697,7 @@ int plan_configure_launch(bgo_plan* P) {
     const size_t per_wave = bk::wave_lds_bytes(nf);
     const size_t lds = per_wave * P->tune.waves_per_block;
     int occ = 0;
-    BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bk::valu_kernel_for(nf), P->threads,
+    BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bk::kernel_for(nf, P->tune.engine), P->threads,
                                                          lds));
     if (occ <= 0) occ = 1;
     if (P->tune.blocks_per_cu > 0) occ = std::min(occ, P->tune.blocks_per_cu);
@@ -683,7 +793,8 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       gnm.push_back(eqT);
       kmax = std::max<uint32_t>(kmax, d.max_mm);
     }
-    while ((gk.size() - F.g_begin) % bk::kUnroll) {  // padding guides never match
+    const size_t pad_to = P->tune.engine == 2 ? 32 : bk::kUnroll;
+    while ((gk.size() - F.g_begin) % pad_to) {  // padding guides never match
       g01.push_back(0xFFF00000u);
       g01.push_back(0xFFF00000u);
       gk.push_back(-1);
@@ -701,6 +812,29 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   const size_t n_pad = gk.size();
   P->n_guides = n_guides;
 
+  // MFMA guide operands: per 32-guide tile, per k-half j and lane l (column
+  // l % 32, lane group l / 32) the 16 int8 of k = 32j + 16(l/32) .. +15 of the
+  // guide's tetrahedral code; padding guides and k >= 60 are 0.
+  P->h_btab.clear();
+  if (P->tune.engine == 2) {
+    P->h_btab.assign(n_pad / 32 * 2 * 64 * 16, 0);
+    for (size_t gi = 0; gi < n_pad; ++gi) {
+      if (gk[gi] < 0) continue;
+      const uint64_t g0 = g01[2 * gi], g1 = g01[2 * gi + 1];
+      const uint64_t bits = g1 | (g0 << 20) | ((g0 ^ g1) << 40);
+      const size_t tile = gi / 32, col = gi % 32;
+      for (int j = 0; j < 2; ++j)
+        for (int grp = 0; grp < 2; ++grp) {
+          int8_t* dst = P->h_btab.data() + ((tile * 2 + j) * 64 + grp * 32 + col) * 16;
+          for (int b = 0; b < 16; ++b) {
+            const int kk = 32 * j + 16 * grp + b;
+            dst[b] = kk < 60 ? (((bits >> kk) & 1u) ? -1 : 1) : 0;
+          }
+        }
+    }
+  }
+  const size_t btab_bytes = P->h_btab.size();
+
   P->n_counter_words = 1 + static_cast<uint32_t>(P->group_first.size()) + n_fam;
   size_t off = 0;
   const size_t o_fam = off; off = align256(off + std::max<size_t>(n_fam, 1) * sizeof(Fam));
@@ -711,6 +845,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   const size_t o_gnm = off; off = align256(off + std::max<size_t>(n_pad, 1) * 8);
   const size_t o_score = off; off = align256(off + 21 * sizeof(double));
   const size_t o_cnt = off; off = align256(off + P->n_counter_words * 8);
+  const size_t o_btab = off; off = align256(off + std::max<size_t>(btab_bytes, 16));
   if (off > P->mem_bytes) {
     const size_t want = std::max(off, P->mem_bytes * 3 / 2);
     if (P->d_mem) (void)hipFree(P->d_mem);
@@ -728,7 +863,10 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   P->d_gnm = reinterpret_cast<uint32_t*>(b + o_gnm);
   P->d_score = reinterpret_cast<double*>(b + o_score);
   P->d_counters = reinterpret_cast<unsigned long long*>(b + o_cnt);
+  P->d_btab = reinterpret_cast<int8_t*>(b + o_btab);
   hipStream_t st = P->stream;
+  if (btab_bytes)
+    BGO_HIP(hipMemcpyAsync(P->d_btab, P->h_btab.data(), btab_bytes, hipMemcpyHostToDevice, st));
   if (n_fam) BGO_HIP(hipMemcpyAsync(P->d_fam, P->fams.data(), n_fam * sizeof(Fam), hipMemcpyHostToDevice, st));
   if (n_pad) {
     BGO_HIP(hipMemcpyAsync(P->d_g01, g01.data(), n_pad * 8, hipMemcpyHostToDevice, st));
@@ -846,11 +984,12 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
     a.interleave = P->tune.reserved[0] ? 1u : 0u;
     a.n_fam = static_cast<uint32_t>(nf);
     a.hit_cap = static_cast<unsigned long long>(P->hit_cap);
-    hipLaunchKernelGGL(bk::valu_kernel_for(nf), dim3(P->group_blocks[gi]), dim3(P->threads),
+    hipLaunchKernelGGL(bk::kernel_for(nf, P->tune.engine), dim3(P->group_blocks[gi]), dim3(P->threads),
                        P->group_lds[gi], P->stream, a, P->d_fam + f0, P->d_g01, P->d_gk, P->d_gaux, P->d_gid,
                        P->d_gnm, P->d_score, P->d_hits, P->d_counters,
                        reinterpret_cast<unsigned int*>(P->d_counters + 1 + gi),
-                       P->d_counters + 1 + P->group_first.size() + f0);
+                       P->d_counters + 1 + P->group_first.size() + f0,
+                       reinterpret_cast<const int4*>(P->d_btab));
     BGO_HIP(hipGetLastError());
   }
   return BGO_OK;

@@ -221,6 +221,21 @@ def main() -> int:
     log(f"batched: {ms_per_step:.2f} ms/step (kernel {kms:.2f} ms), {value:.1f} guide·Gbp/s, "
         f"{n} sites, families {families}")
 
+    # both compare engines on the same workload (same tiling otherwise)
+    engines = {}
+    for eng, name in ((1, "valu"), (2, "mfma")):
+        t_eng = dict(plan.tuning())
+        t_eng["engine"] = eng
+        p2 = Plan(dg, specs, tuning=t_eng, hit_capacity=default_capacity(len(specs)))
+        times = []
+        for _ in range(3):
+            p2.launch()
+            n2, ms2 = p2.wait()
+            times.append(ms2)
+        engines[name] = {"kernel_ms": round(min(times[1:]), 3), "sites": int(n2)}
+        p2.close()
+    log(f"engines: {engines}")
+
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "guide·Gbp/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
@@ -228,9 +243,11 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": w.name, "description": w.description, "genome_bp": L,
                    "guides": len(specs), "max_mismatches": args.max_mm,
-                   "pams": sorted({s.pam for s in specs}), "engine": "batched VALU",
+                   "pams": sorted({s.pam for s in specs}),
+                   "engine": {1: "batched VALU", 2: "batched int8 MFMA"}[plan.tuning()["engine"]],
                    "graph": not args.eager, "tuning": plan.tuning()},
         "sites": int(n),
+        "engines": engines,
         "autotune": tune_trials,
         "kernel_ms": round(kms, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),

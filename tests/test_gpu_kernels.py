@@ -105,18 +105,23 @@ def test_plain_small_config(gpu):
     assert plain_set(w.genome, guides, 3) == oracle_set(w.genome, guides, 3)
 
 
+ENGINES = [1, 2]   # 1: VALU xor/popcount compare, 2: int8 MFMA compare
+
+
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("seed", range(8))
-def test_batched_matches_oracle_small(gpu, seed):
+def test_batched_matches_oracle_small(gpu, seed, engine):
     contigs, guides, max_mm = build_case(seed)
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"]) if "N" in "".join(
         s for _, s in contigs) else DeviceGenome(genome)
     nm = genome if dg.has_n_mask else {k: v for k, v in genome.items() if not k.startswith("n_mask")}
-    r = scan(dg, [GuideSpec(s, p, max_mm) for s, p in guides])
+    r = scan(dg, [GuideSpec(s, p, max_mm) for s, p in guides], tuning={"engine": engine})
     assert hit_set(r.hits) == oracle_set(nm, guides, max_mm)
 
 
-def test_batched_mixed_budgets_many_families_ranges(gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_batched_mixed_budgets_many_families_ranges(gpu, engine):
     """>8 PAM families (two launch groups), per-guide budgets, 20/21-nt
     guides, position ranges and single-strand launches."""
     rng = np.random.default_rng(3)
@@ -127,25 +132,26 @@ def test_batched_mixed_budgets_many_families_ranges(gpu):
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
-    r = scan(dg, specs)
+    r = scan(dg, specs, tuning={"engine": engine})
     assert len(r.families) == 12
     assert hit_set(r.hits) == oracle_set(genome, guides, None, mm_per=budgets)
     L = genome["total_bp"]
     for (b, e), strands in [((1000, 17_000), 3), ((0, L // 2), 1), ((L // 3, L), 2)]:
-        r = scan(dg, specs, pos_begin=b, pos_end=e, strands=strands)
+        r = scan(dg, specs, pos_begin=b, pos_end=e, strands=strands, tuning={"engine": engine})
         want = oracle_set(genome, guides, None, mm_per=budgets, pos=(b, e),
                           strands=[s for s in (0, 1) if strands & (1 << s)])
         assert hit_set(r.hits) == want
 
 
-def test_batched_capacity_rerun_and_fused_epilogue(gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_batched_capacity_rerun_and_fused_epilogue(gpu, engine):
     rng = np.random.default_rng(9)
     contigs, guides, max_mm = build_case(31, length=(50_000, 80_000), n_guides=6, max_mm=5)
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     specs = [GuideSpec(s, p, 6, ["ABE", "CBE"][i % 2], (int(rng.integers(1, 6)), int(rng.integers(6, 21))))
              for i, (s, p) in enumerate(guides)]
-    r = scan(dg, specs, hit_capacity=3)
+    r = scan(dg, specs, hit_capacity=3, tuning={"engine": engine})
     assert r.launches == 2 and r.n_hits == r.hits.size > 3
     assert hit_set(r.hits) == oracle_set(genome, guides, 6)
     h = r.hits
@@ -168,13 +174,14 @@ def test_batched_capacity_rerun_and_fused_epilogue(gpu):
             assert bool(row["editable"]) == bool(editable_base_in_window(seq, sp.editor_type, *sp.window))
 
 
-def test_batched_family_window_counts(gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_batched_family_window_counts(gpu, engine):
     contigs, guides, max_mm = build_case(5, length=(5000, 9000))
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     pams = ["NGG", "NNNRRT", "NG", "TTTV"]
     specs = [GuideSpec(guides[0][0], p, 3) for p in pams]
-    r = scan(dg, specs)
+    r = scan(dg, specs, tuning={"engine": engine})
     L = genome["total_bp"]
     from beguidedoff_amd.pam import IUPAC_MASK
     for (n_g, windows), pam in zip(r.families, pams):
@@ -188,12 +195,13 @@ def test_batched_family_window_counts(gpu):
         assert n_g == 1 and windows == want
 
 
-def test_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_graph_replays_match_eager(gpu, engine):
     contigs, guides, max_mm = build_case(8, length=(30_000, 50_000), n_guides=8)
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     specs = [GuideSpec(s, p, max_mm) for s, p in guides]
-    plan = Plan(dg, specs, hit_capacity=1 << 16)
+    plan = Plan(dg, specs, hit_capacity=1 << 16, tuning={"engine": engine})
     plan.launch()
     n, _ = plan.wait()
     eager = hit_set(plan.fetch(n))
@@ -217,7 +225,9 @@ def test_tunings_give_identical_sites(gpu):
     want = oracle_set(genome, guides, max_mm)
     for t in ({"chunk_positions": 64}, {"chunk_positions": 4096, "interleave": 1},
               {"chunk_positions": 1 << 20, "waves_per_block": 1, "blocks_per_cu": 1},
-              {"waves_per_block": 2, "blocks_per_cu": 3, "interleave": 1}):
+              {"waves_per_block": 2, "blocks_per_cu": 3, "interleave": 1},
+              {"engine": 2, "chunk_positions": 64}, {"engine": 2, "waves_per_block": 1},
+              {"engine": 2, "chunk_positions": 1 << 20, "interleave": 1}):
         r = scan(dg, specs, tuning=t)
         assert hit_set(r.hits) == want, t
     from beguidedoff_amd.autotune import autotune
@@ -227,3 +237,19 @@ def test_tunings_give_identical_sites(gpu):
                             space={"chunk_positions": [4096, 65536], "waves_per_block": [4],
                                    "blocks_per_cu": [0], "interleave": [0, 1]})
     assert len(trials) == 4 and hit_set(scan(dg, specs, tuning=best).hits) == want
+
+
+def test_mfma_engine_many_guides_high_budget(gpu):
+    """Hundreds of guides per family (many 32-guide B tiles), budgets up to 8
+    so candidates are frequent, N runs and planted sites on both strands."""
+    rng = np.random.default_rng(17)
+    contigs, guides, _ = build_case(99, length=(30_000, 50_000), n_guides=4)
+    seqs = ["".join(rng.choice(list("ACGT"), size=20)) for _ in range(150)]
+    guides = [(q, ["NGG", "NG", "NNNRRT"][i % 3]) for i, q in enumerate(seqs)] + guides
+    budgets = [int(rng.integers(0, 9)) for _ in guides]
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
+    want = oracle_set(genome, guides, None, mm_per=budgets)
+    for engine in ENGINES:
+        assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
