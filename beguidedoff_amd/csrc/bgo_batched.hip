@@ -239,14 +239,67 @@ __device__ inline v4i expand16(uint32_t bits) {
   return r;
 }
 
+// One 32-guide tile against the (up to) two 32-window A tiles.
+__device__ inline void mfma_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
+                                 uint32_t head, uint32_t n, uint32_t cs, uint32_t lane, uint32_t m,
+                                 uint32_t g, bool two, const v4i (&A)[2][2], int4 b0, int4 b1,
+                                 uint32_t gt) {
+  const v16i zero = {};
+  const v4i B0 = {b0.x, b0.y, b0.z, b0.w}, B1 = {b1.x, b1.y, b1.z, b1.w};
+  // k 60..63 of the guide operand hold kmax - 15 (the window's are +1), so
+  // each dot product arrives biased by 4*kmax - 60.
+  v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B0, zero, 0, 0, 0);
+  acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][1], B1, acc0, 0, 0, 0);
+  v16i acc1 = zero - 1;
+  if (two) {
+    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, zero, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
+  }
+  uint32_t all = static_cast<uint32_t>(acc0[0]);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) all &= static_cast<uint32_t>(acc0[r]);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) all &= static_cast<uint32_t>(acc1[r]);
+  if (!ballot(static_cast<int>(all) >= 0)) return;
+  // Rare: entries with a non-negative accumulator -> exact per-pair check.
+  uint32_t bits = 0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    bits |= static_cast<uint32_t>(acc0[r] >= 0) << r;
+    bits |= static_cast<uint32_t>(acc1[r] >= 0) << (16 + r);
+  }
+  while (ballot(bits != 0)) {
+    bool cand = false;
+    uint32_t gi = 0, pos = 0, W0 = 0, W1 = 0, c = 0;
+    if (bits) {
+      const uint32_t bit = __builtin_ctz(bits);
+      bits &= bits - 1;
+      const uint32_t r = bit & 15u, tile = bit >> 4;
+      const uint32_t row = 8u * (r >> 2) + 4u * g + (r & 3u);
+      const uint32_t widx = 32u * tile + row;
+      gi = gt + m;
+      if (widx < n) {
+        const uint2 e = L[(head + widx) & (kList - 1)];
+        W0 = e.x & kWin;
+        W1 = e.y & kWin;
+        pos = cs + ((e.x >> 20) | ((e.y >> 20) << 12));
+        c = __popc((W0 ^ t.g01[2 * gi]) | (W1 ^ t.g01[2 * gi + 1]));
+        cand = static_cast<int>(c) <= t.gk[gi];
+      }
+    }
+    if (ballot(cand)) record_candidates(a, t, s, gi, pos, W0, W1, c, cand, lane);
+  }
+}
+
 // int8 MFMA engine of compare_list.  Each window and guide is a 64-element
 // int8 vector: for every base the tetrahedral code (+-1 per plane of b1, b0,
 // b0^b1; elements 60..63 are 0 in the guide operand), so a window-guide dot
-// product is 60 - 4 * (2-bit mismatches).  One wave multiplies up to 64
+// product is 60 - 4 * (2-bit mismatches) + 4 * (kmax - 15).  One wave multiplies up to 64
 // windows (two 32-row A tiles, one window per lane pair) by 32-guide B tiles
-// with v_mfma_i32_32x32x32_i8 (K = 2 x 32).  Accumulators start at
-// 4*kmax - 60, so a non-negative entry marks a candidate with <= kmax raw
-// mismatches; the common "none" case is one AND-reduction of sign bits.
+// with v_mfma_i32_32x32x32_i8 (K = 2 x 32).  A non-negative result marks a
+// candidate with <= kmax raw
+// mismatches (the bias rides in k = 60..63); the common "none" case is one
+// AND-reduction of sign bits.
 // Lane layouts (checked on gfx950): A/B row or column = lane % 32, lane group
 // lane / 32 selects 16 of the 32 k of each instruction; D register r of lane
 // l is row 8*(r/4) + 4*(l/32) + r%4, column l % 32.
@@ -267,56 +320,16 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
     A[tile][0] = expand16((lo >> (16 * g)) & 0xFFFFu);
     A[tile][1] = expand16((hi >> (16 * g)) & 0xFFFFu);
   }
-  const int bias = 4 * static_cast<int>(F.kmax) - 60;
-  v16i cinit;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) cinit[r] = bias;
   const bool two = n > 32;
-  for (uint32_t gt = F.g_begin; gt < F.g_end; gt += 32) {
-    const int4* __restrict__ bp = t.btab + (gt / 32) * 128;
-    const int4 b0 = bp[lane], b1 = bp[64 + lane];
-    const v4i B0 = {b0.x, b0.y, b0.z, b0.w}, B1 = {b1.x, b1.y, b1.z, b1.w};
-    v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B0, cinit, 0, 0, 0);
-    acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][1], B1, acc0, 0, 0, 0);
-    v16i acc1 = cinit;
-    if (two) {
-      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, cinit, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
-    }
-    uint32_t all = static_cast<uint32_t>(acc0[0]);
-#pragma unroll
-    for (int r = 1; r < 16; ++r) all &= static_cast<uint32_t>(acc0[r]);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) all &= static_cast<uint32_t>(acc1[r]);
-    if (!ballot(static_cast<int>(all) >= 0)) continue;
-    // Rare: entries with a non-negative accumulator -> exact per-pair check.
-    uint32_t bits = 0;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      bits |= static_cast<uint32_t>(acc0[r] >= 0) << r;
-      bits |= static_cast<uint32_t>(acc1[r] >= 0) << (16 + r);
-    }
-    while (ballot(bits != 0)) {
-      bool cand = false;
-      uint32_t gi = 0, pos = 0, W0 = 0, W1 = 0, c = 0;
-      if (bits) {
-        const uint32_t bit = __builtin_ctz(bits);
-        bits &= bits - 1;
-        const uint32_t r = bit & 15u, tile = bit >> 4;
-        const uint32_t row = 8u * (r >> 2) + 4u * g + (r & 3u);
-        const uint32_t widx = 32u * tile + row;
-        gi = gt + m;
-        if (widx < n) {
-          const uint2 e = L[(head + widx) & (kList - 1)];
-          W0 = e.x & kWin;
-          W1 = e.y & kWin;
-          pos = cs + ((e.x >> 20) | ((e.y >> 20) << 12));
-          c = __popc((W0 ^ t.g01[2 * gi]) | (W1 ^ t.g01[2 * gi + 1]));
-          cand = static_cast<int>(c) <= t.gk[gi];
-        }
-      }
-      if (ballot(cand)) record_candidates(a, t, s, gi, pos, W0, W1, c, cand, lane);
-    }
+  const int4* __restrict__ bp = t.btab + (F.g_begin / 32) * 128;
+  // The next tile's guide operands load while this tile computes; the table
+  // ends with one spare all-zero tile, so the look-ahead load stays in bounds.
+  int4 b0 = bp[lane], b1 = bp[64 + lane];
+  for (uint32_t gt = F.g_begin; gt < F.g_end; gt += 32, bp += 128) {
+    const int4 c0 = b0, c1 = b1;
+    b0 = bp[128 + lane];
+    b1 = bp[192 + lane];
+    mfma_tile(a, t, s, L, head, n, cs, lane, m, g, two, A, c0, c1, gt);
   }
 }
 
@@ -814,23 +827,27 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
 
   // MFMA guide operands: per 32-guide tile, per k-half j and lane l (column
   // l % 32, lane group l / 32) the 16 int8 of k = 32j + 16(l/32) .. +15 of the
-  // guide's tetrahedral code; padding guides and k >= 60 are 0.
+  // guide's tetrahedral code; k 60..63 hold the family's kmax - 15 (the
+  // candidate bias); padding guides are all 0.
   P->h_btab.clear();
   if (P->tune.engine == 2) {
-    P->h_btab.assign(n_pad / 32 * 2 * 64 * 16, 0);
-    for (size_t gi = 0; gi < n_pad; ++gi) {
-      if (gk[gi] < 0) continue;
-      const uint64_t g0 = g01[2 * gi], g1 = g01[2 * gi + 1];
-      const uint64_t bits = g1 | (g0 << 20) | ((g0 ^ g1) << 40);
-      const size_t tile = gi / 32, col = gi % 32;
-      for (int j = 0; j < 2; ++j)
-        for (int grp = 0; grp < 2; ++grp) {
-          int8_t* dst = P->h_btab.data() + ((tile * 2 + j) * 64 + grp * 32 + col) * 16;
-          for (int b = 0; b < 16; ++b) {
-            const int kk = 32 * j + 16 * grp + b;
-            dst[b] = kk < 60 ? (((bits >> kk) & 1u) ? -1 : 1) : 0;
+    P->h_btab.assign((n_pad / 32 + 1) * 2 * 64 * 16, 0);  // + spare look-ahead tile
+    for (const Fam& F : P->fams) {
+      const int8_t bias = static_cast<int8_t>(static_cast<int>(F.kmax) - 15);
+      for (size_t gi = F.g_begin; gi < F.g_end; ++gi) {
+        if (gk[gi] < 0) continue;
+        const uint64_t g0 = g01[2 * gi], g1 = g01[2 * gi + 1];
+        const uint64_t bits = g1 | (g0 << 20) | ((g0 ^ g1) << 40);
+        const size_t tile = gi / 32, col = gi % 32;
+        for (int j = 0; j < 2; ++j)
+          for (int grp = 0; grp < 2; ++grp) {
+            int8_t* dst = P->h_btab.data() + ((tile * 2 + j) * 64 + grp * 32 + col) * 16;
+            for (int b = 0; b < 16; ++b) {
+              const int kk = 32 * j + 16 * grp + b;
+              dst[b] = kk < 60 ? (((bits >> kk) & 1u) ? -1 : 1) : bias;
+            }
           }
-        }
+      }
     }
   }
   const size_t btab_bytes = P->h_btab.size();
