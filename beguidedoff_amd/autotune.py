@@ -1,6 +1,6 @@
 """Autotuner of the batched kernel's launch shape.
 
-Tunables (bgo_tuning): compare engine (VALU xor/popcount or int8 MFMA),
+Tunables (bgo_tuning): compare engine (VALU xor/popcount, int8 or fp4 MFMA),
 positions per work item (genome tile length), waves per block, resident
 blocks per CU, and strand interleave of the work items.
 Each candidate runs the real kernel on a slice of the resident genome with
@@ -21,7 +21,7 @@ from . import _lib
 from .batched import DeviceGenome, GuideSpec, Plan
 
 DEFAULT_SPACE = {
-    "engine": [1, 2],
+    "engine": [1, 2, 3],
     "chunk_positions": [32768, 131072],
     "waves_per_block": [2, 4],
     "blocks_per_cu": [0],

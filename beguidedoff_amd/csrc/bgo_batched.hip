@@ -85,7 +85,7 @@ struct Tab {
   const double* __restrict__ score_tab;  // [21]
   bgo_hit* __restrict__ hits;
   unsigned long long* __restrict__ hit_count;
-  const int4* __restrict__ btab;         // MFMA engine: [guide tile][j][lane] int8 x16
+  const int4* __restrict__ btab;         // MFMA engines: guide operand tiles
 };
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
@@ -220,13 +220,36 @@ __device__ inline uint32_t allowed(uint32_t nib, uint32_t x0, uint32_t x1) {
   }
 }
 
-// Stream positions [cs, ce) of strand s.  Each lane owns a 32-position block:
-// the PAM test of every family runs on whole plane words (one allowed-set op
-// per PAM position for 32 positions), then the passing windows are appended
-// to the family's ring buffer in LDS; every 64 buffered windows are compared
-// against the family's guides (one window per lane).
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// MFMA engines of compare_list.  Windows and guides are coded as 64-element
+// vectors over the tetrahedral code of the 2-bit bases: one element for each
+// of b1, b0 and b0^b1 of every base (elements 0..19, 20..39, 40..59).  Two
+// bases agree in all three elements or differ in exactly two, so the codes of
+// a window and a guide differ in 2 x (2-bit mismatches) elements.  Elements
+// 60..63 carry a bias that turns "within budget" into the sign of the
+// product.  One wave multiplies up to 64 buffered windows (two 32-row A
+// tiles, row = lane % 32) by 32-guide B tiles (column = lane % 32); a sign-bit
+// AND-reduction over the lane's 2 x 16 accumulators settles the common "no
+// site" case, and non-negative entries take the exact path.
+//  * ENGINE 2, int8 (v_mfma_i32_32x32x32_i8, two steps of K = 32): elements
+//    are +1 (bit 0) / -1 (bit 1); guide elements 60..63 hold kmax - 15 and
+//    window ones +1, so a product is 4 * (kmax - mismatches).
+//  * ENGINE 3, fp4 e2m1 (v_mfma_scale_f32_32x32x64_f8f6f4, K = 64 in one
+//    step, twice the int8 rate): window elements are 0.5 * bit, guide
+//    elements -1 (bit 0) / +1 (bit 1).  With window bits a and guide bits b,
+//    sum(a (2b - 1)) = |b| - 2 mm and |b| = 2 x (non-A guide bases), so the
+//    code part is (non-A guide bases) - mm; guide elements 60..63 sum to
+//    budget - (non-A guide bases) against window elements of 1.0, and the
+//    f32 product is exactly budget - mm.
+// Padding guides get a negative bias, so they never reach the exact path.
+// Lane layouts (checked on gfx950): A/B row or column = lane % 32; lane group
+// lane / 32 holds half of K and the hardware pairs equal (group, slot) of A
+// and B; D register r of lane l is row 8*(r/4) + 4*(l/32) + r%4, column
+// l % 32.
 
 // 16 bits -> 16 int8 of +1 (bit 0) / -1 (bit 1), 4 per dword.
 __device__ inline v4i expand16(uint32_t bits) {
@@ -239,35 +262,20 @@ __device__ inline v4i expand16(uint32_t bits) {
   return r;
 }
 
-// One 32-guide tile against the (up to) two 32-window A tiles.
-__device__ inline void mfma_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                 uint32_t head, uint32_t n, uint32_t cs, uint32_t lane, uint32_t m,
-                                 uint32_t g, bool two, const v4i (&A)[2][2], int4 b0, int4 b1,
-                                 uint32_t gt) {
-  const v16i zero = {};
-  const v4i B0 = {b0.x, b0.y, b0.z, b0.w}, B1 = {b1.x, b1.y, b1.z, b1.w};
-  // k 60..63 of the guide operand hold kmax - 15 (the window's are +1), so
-  // each dot product arrives biased by 4*kmax - 60.
-  v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B0, zero, 0, 0, 0);
-  acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][1], B1, acc0, 0, 0, 0);
-  v16i acc1 = zero - 1;
-  if (two) {
-    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, zero, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
-  }
-  uint32_t all = static_cast<uint32_t>(acc0[0]);
-#pragma unroll
-  for (int r = 1; r < 16; ++r) all &= static_cast<uint32_t>(acc0[r]);
-#pragma unroll
-  for (int r = 0; r < 16; ++r) all &= static_cast<uint32_t>(acc1[r]);
-  if (!ballot(static_cast<int>(all) >= 0)) return;
-  // Rare: entries with a non-negative accumulator -> exact per-pair check.
-  uint32_t bits = 0;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    bits |= static_cast<uint32_t>(acc0[r] >= 0) << r;
-    bits |= static_cast<uint32_t>(acc1[r] >= 0) << (16 + r);
-  }
+// 8 bits -> 8 fp4 nibbles of 0.5 * bit (bit i -> nibble i).
+__device__ inline uint32_t spread8(uint32_t b) {
+  b = (b | (b << 12)) & 0x000F000Fu;
+  b = (b | (b << 6)) & 0x03030303u;
+  return (b | (b << 3)) & 0x11111111u;
+}
+
+// Exact check of the (window, guide) pairs of one guide tile whose
+// accumulator is non-negative: bit r (16 + r) of `bits` = register r of the
+// product with A tile 0 (1).
+__device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const uint2* L,
+                                  uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
+                                  uint32_t bits, uint32_t gt) {
+  const uint32_t m = lane & 31u, g = lane >> 5;
   while (ballot(bits != 0)) {
     bool cand = false;
     uint32_t gi = 0, pos = 0, W0 = 0, W1 = 0, c = 0;
@@ -291,45 +299,109 @@ __device__ inline void mfma_tile(const Args& a, const Tab& t, uint32_t s, const 
   }
 }
 
-// int8 MFMA engine of compare_list.  Each window and guide is a 64-element
-// int8 vector: for every base the tetrahedral code (+-1 per plane of b1, b0,
-// b0^b1; elements 60..63 are 0 in the guide operand), so a window-guide dot
-// product is 60 - 4 * (2-bit mismatches) + 4 * (kmax - 15).  One wave multiplies up to 64
-// windows (two 32-row A tiles, one window per lane pair) by 32-guide B tiles
-// with v_mfma_i32_32x32x32_i8 (K = 2 x 32).  A non-negative result marks a
-// candidate with <= kmax raw
-// mismatches (the bias rides in k = 60..63); the common "none" case is one
-// AND-reduction of sign bits.
-// Lane layouts (checked on gfx950): A/B row or column = lane % 32, lane group
-// lane / 32 selects 16 of the 32 k of each instruction; D register r of lane
-// l is row 8*(r/4) + 4*(l/32) + r%4, column l % 32.
+// Sign-bit test of a guide tile's two products (as 32-bit patterns): the
+// common case returns after one AND-reduction and one ballot.
+__device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const uint2* L,
+                                  uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
+                                  const v16i& r0, const v16i& r1, uint32_t gt) {
+  uint32_t all = static_cast<uint32_t>(r0[0]);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) all &= static_cast<uint32_t>(r0[r]);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) all &= static_cast<uint32_t>(r1[r]);
+  if (!ballot(static_cast<int>(all) >= 0)) return;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    bits |= static_cast<uint32_t>(r0[r] >= 0) << r;
+    bits |= static_cast<uint32_t>(r1[r] >= 0) << (16 + r);
+  }
+  mfma_exact(a, t, s, L, head, n, cs, lane, bits, gt);
+}
+
+// int8: one 32-guide tile against the (up to) two 32-window A tiles.
+__device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
+                               uint32_t head, uint32_t n, uint32_t cs, uint32_t lane, bool two,
+                               const v4i (&A)[2][2], int4 b0, int4 b1, uint32_t gt) {
+  const v16i zero = {};
+  const v4i B0 = {b0.x, b0.y, b0.z, b0.w}, B1 = {b1.x, b1.y, b1.z, b1.w};
+  v16i acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][0], B0, zero, 0, 0, 0);
+  acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[0][1], B1, acc0, 0, 0, 0);
+  v16i acc1 = zero - 1;
+  if (two) {
+    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, zero, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
+  }
+  mfma_signs(a, t, s, L, head, n, cs, lane, acc0, acc1, gt);
+}
+
+// fp4: one 32-guide tile (one K = 64 step per A tile).  Scales are 2^0.
+__device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
+                                uint32_t head, uint32_t n, uint32_t cs, uint32_t lane, bool two,
+                                const v8i (&A)[2], uint4 b, uint32_t gt) {
+  const v16f zero = {};
+  const v8i B = {static_cast<int>(b.x), static_cast<int>(b.y), static_cast<int>(b.z),
+                 static_cast<int>(b.w), 0, 0, 0, 0};
+  const v16f p0 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[0], B, zero, 4, 4, 0, 127, 0,
+                                                                  127);
+  v16i r1 = v16i{} - 1;
+  if (two)
+    r1 = __builtin_bit_cast(
+        v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[1], B, zero, 4, 4, 0, 127, 0, 127));
+  mfma_signs(a, t, s, L, head, n, cs, lane, __builtin_bit_cast(v16i, p0), r1, gt);
+}
+
+// Compare the first n (<= 64) windows of a family list with all guides of the
+// family on the matrix cores.  The guide operand table ends with one spare
+// all-zero tile, so the look-ahead load of the next tile stays in bounds.
+template <int ENGINE>
 __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam& F, uint32_t s,
                                          const uint2* L, uint32_t head, uint32_t n, uint32_t cs,
                                          uint32_t lane) {
   const uint32_t m = lane & 31u, g = lane >> 5;
-  v4i A[2][2];
+  const bool two = n > 32;
+  uint32_t lo[2], hi[2];  // code elements 0..31 and 32..63 of window rows m, 32 + m
 #pragma unroll
   for (int tile = 0; tile < 2; ++tile) {
     const uint32_t widx = 32u * tile + m;
     uint2 e = make_uint2(0u, 0u);
     if (widx < n) e = L[(head + widx) & (kList - 1)];
     const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
-    // 64-bit string: W1 at bits 0..19, W0 at 20..39, W0^W1 at 40..59
-    const uint32_t lo = W1 | (W0 << 20);
-    const uint32_t hi = (W0 >> 12) | ((W0 ^ W1) << 8);
-    A[tile][0] = expand16((lo >> (16 * g)) & 0xFFFFu);
-    A[tile][1] = expand16((hi >> (16 * g)) & 0xFFFFu);
+    lo[tile] = W1 | (W0 << 20);
+    hi[tile] = (W0 >> 12) | ((W0 ^ W1) << 8);
   }
-  const bool two = n > 32;
-  const int4* __restrict__ bp = t.btab + (F.g_begin / 32) * 128;
-  // The next tile's guide operands load while this tile computes; the table
-  // ends with one spare all-zero tile, so the look-ahead load stays in bounds.
-  int4 b0 = bp[lane], b1 = bp[64 + lane];
-  for (uint32_t gt = F.g_begin; gt < F.g_end; gt += 32, bp += 128) {
-    const int4 c0 = b0, c1 = b1;
-    b0 = bp[128 + lane];
-    b1 = bp[192 + lane];
-    mfma_tile(a, t, s, L, head, n, cs, lane, m, g, two, A, c0, c1, gt);
+  if constexpr (ENGINE == 3) {
+    v8i A[2];
+    const uint32_t pad = g ? 0x22220000u : 0u;  // elements 60..63 = 1.0
+#pragma unroll
+    for (int tile = 0; tile < 2; ++tile) {
+      const uint32_t c = g ? hi[tile] : lo[tile];
+      A[tile] = v8i{static_cast<int>(spread8(c & 0xFFu)), static_cast<int>(spread8((c >> 8) & 0xFFu)),
+                    static_cast<int>(spread8((c >> 16) & 0xFFu)),
+                    static_cast<int>(spread8(c >> 24) | pad), 0, 0, 0, 0};
+    }
+    const uint4* __restrict__ bp = reinterpret_cast<const uint4*>(t.btab) + (F.g_begin / 32) * 64;
+    uint4 b = bp[lane];
+    for (uint32_t gt = F.g_begin; gt < F.g_end; gt += 32, bp += 64) {
+      const uint4 c = b;
+      b = bp[64 + lane];
+      fp4_tile(a, t, s, L, head, n, cs, lane, two, A, c, gt);
+    }
+  } else {
+    v4i A[2][2];
+#pragma unroll
+    for (int tile = 0; tile < 2; ++tile) {
+      A[tile][0] = expand16((lo[tile] >> (16 * g)) & 0xFFFFu);
+      A[tile][1] = expand16((hi[tile] >> (16 * g)) & 0xFFFFu);
+    }
+    const int4* __restrict__ bp = t.btab + (F.g_begin / 32) * 128;
+    int4 b0 = bp[lane], b1 = bp[64 + lane];
+    for (uint32_t gt = F.g_begin; gt < F.g_end; gt += 32, bp += 128) {
+      const int4 c0 = b0, c1 = b1;
+      b0 = bp[128 + lane];
+      b1 = bp[192 + lane];
+      i8_tile(a, t, s, L, head, n, cs, lane, two, A, c0, c1, gt);
+    }
   }
 }
 
@@ -368,8 +440,8 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
     }
     wave_sync();
     if (tail - head < 64) break;
-    if (ENGINE == 2)
-      compare_list_mfma(a, t, t.fam[f], s, L, head, 64, cs, lane);
+    if constexpr (ENGINE >= 2)
+      compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, 64, cs, lane);
     else
       compare_list(a, t, t.fam[f], s, L, head, 64, cs, lane);
     head += 64;
@@ -442,8 +514,8 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
     const uint32_t head = uniform(w.heads[f]);
     const uint32_t c = uniform(w.tails[f]) - head;
     if (c) {
-      if (ENGINE == 2)
-        compare_list_mfma(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
+      if constexpr (ENGINE >= 2)
+        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
       else
         compare_list(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
       if (lane == 0) w.wstats[f] += c;
@@ -453,7 +525,7 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
 }
 
 template <int NF, int ENGINE>
-__global__ __launch_bounds__(kMaxBlock) void batched_valu_kernel(
+__global__ __launch_bounds__(kMaxBlock) void batched_scan_kernel(
     Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
     const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
     const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gnm,
@@ -505,19 +577,19 @@ using KernelFn = void (*)(Args, const Fam*, const uint32_t*, const int32_t*, con
 template <int ENGINE>
 inline KernelFn kernel_for_nf(int nf) {
   switch (nf) {
-    case 1: return batched_valu_kernel<1, ENGINE>;
-    case 2: return batched_valu_kernel<2, ENGINE>;
-    case 3: return batched_valu_kernel<3, ENGINE>;
-    case 4: return batched_valu_kernel<4, ENGINE>;
-    case 5: return batched_valu_kernel<5, ENGINE>;
-    case 6: return batched_valu_kernel<6, ENGINE>;
-    case 7: return batched_valu_kernel<7, ENGINE>;
-    default: return batched_valu_kernel<8, ENGINE>;
+    case 1: return batched_scan_kernel<1, ENGINE>;
+    case 2: return batched_scan_kernel<2, ENGINE>;
+    case 3: return batched_scan_kernel<3, ENGINE>;
+    case 4: return batched_scan_kernel<4, ENGINE>;
+    case 5: return batched_scan_kernel<5, ENGINE>;
+    case 6: return batched_scan_kernel<6, ENGINE>;
+    case 7: return batched_scan_kernel<7, ENGINE>;
+    default: return batched_scan_kernel<8, ENGINE>;
   }
 }
 
 inline KernelFn kernel_for(int nf, int engine) {
-  return engine == 2 ? kernel_for_nf<2>(nf) : kernel_for_nf<1>(nf);
+  return engine == 3 ? kernel_for_nf<3>(nf) : engine == 2 ? kernel_for_nf<2>(nf) : kernel_for_nf<1>(nf);
 }
 
 __global__ void reset_counters_kernel(unsigned long long* c, uint32_t n) {
@@ -682,6 +754,19 @@ Fam make_fam(const bgo_guide_desc& d) {
   return F;
 }
 
+// Four fp4 (e2m1) codes summing to x, |x| <= 22 (magnitudes 6, 4, 3, 2, 1).
+bool fp4_sum4(int x, uint8_t out[4]) {
+  const uint8_t sign = x < 0 ? 0x8 : 0x0;
+  int r = x < 0 ? -x : x;
+  for (int i = 0; i < 4; ++i) {
+    const int v = r >= 6 ? 6 : (r == 5 ? 4 : r);
+    static const uint8_t code[7] = {0x0, 0x2, 0x4, 0x5, 0x6, 0x0, 0x7};  // 0 1 2 3 4 - 6
+    out[i] = v ? static_cast<uint8_t>(code[v] | sign) : 0;
+    r -= v;
+  }
+  return r == 0;
+}
+
 int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   bgo_tuning r{};
   if (t) r = *t;
@@ -692,8 +777,8 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
     return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");
   if (r.engine <= 0) r.engine = 1;
-  if (r.engine != 1 && r.engine != 2) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
-  r.guide_unroll = r.engine == 2 ? 32 : bk::kUnroll;
+  if (r.engine < 1 || r.engine > 3) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
+  r.guide_unroll = r.engine >= 2 ? 32 : bk::kUnroll;
   P->tune = r;
   return BGO_OK;
 }
@@ -806,7 +891,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       gnm.push_back(eqT);
       kmax = std::max<uint32_t>(kmax, d.max_mm);
     }
-    const size_t pad_to = P->tune.engine == 2 ? 32 : bk::kUnroll;
+    const size_t pad_to = P->tune.engine >= 2 ? 32 : bk::kUnroll;
     while ((gk.size() - F.g_begin) % pad_to) {  // padding guides never match
       g01.push_back(0xFFF00000u);
       g01.push_back(0xFFF00000u);
@@ -825,28 +910,58 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   const size_t n_pad = gk.size();
   P->n_guides = n_guides;
 
-  // MFMA guide operands: per 32-guide tile, per k-half j and lane l (column
-  // l % 32, lane group l / 32) the 16 int8 of k = 32j + 16(l/32) .. +15 of the
-  // guide's tetrahedral code; k 60..63 hold the family's kmax - 15 (the
-  // candidate bias); padding guides are all 0.
+  // MFMA guide operands (each table ends with one spare all-zero tile for the
+  // kernel's look-ahead load).
+  //  * int8: per 32-guide tile, per k-half j and lane l (column l % 32, lane
+  //    group l / 32) the 16 int8 of k = 32j + 16(l/32) .. +15 of the guide's
+  //    tetrahedral code; k 60..63 hold the family's kmax - 15; padding
+  //    guides hold -127 there (never a candidate).
+  //  * fp4: per 32-guide tile and lane l the 32 nibbles of elements
+  //    32(l/32) .. +31 (nibble i of dword i/8 at bits 4(i%8)); elements 60..63
+  //    sum to budget - (non-A bases); padding guides are 0 with -6 x 4 there.
   P->h_btab.clear();
   if (P->tune.engine == 2) {
-    P->h_btab.assign((n_pad / 32 + 1) * 2 * 64 * 16, 0);  // + spare look-ahead tile
+    P->h_btab.assign((n_pad / 32 + 1) * 2 * 64 * 16, 0);
     for (const Fam& F : P->fams) {
       const int8_t bias = static_cast<int8_t>(static_cast<int>(F.kmax) - 15);
       for (size_t gi = F.g_begin; gi < F.g_end; ++gi) {
-        if (gk[gi] < 0) continue;
+        const bool real = gk[gi] >= 0;
         const uint64_t g0 = g01[2 * gi], g1 = g01[2 * gi + 1];
-        const uint64_t bits = g1 | (g0 << 20) | ((g0 ^ g1) << 40);
+        const uint64_t bits = real ? (g1 | (g0 << 20) | ((g0 ^ g1) << 40)) : 0;
         const size_t tile = gi / 32, col = gi % 32;
         for (int j = 0; j < 2; ++j)
           for (int grp = 0; grp < 2; ++grp) {
             int8_t* dst = P->h_btab.data() + ((tile * 2 + j) * 64 + grp * 32 + col) * 16;
             for (int b = 0; b < 16; ++b) {
               const int kk = 32 * j + 16 * grp + b;
-              dst[b] = kk < 60 ? (((bits >> kk) & 1u) ? -1 : 1) : bias;
+              if (kk >= 60)
+                dst[b] = real ? bias : static_cast<int8_t>(-127);
+              else
+                dst[b] = real ? (((bits >> kk) & 1u) ? -1 : 1) : 0;
             }
           }
+      }
+    }
+  } else if (P->tune.engine == 3) {
+    P->h_btab.assign((n_pad / 32 + 1) * 64 * 16, 0);
+    for (const Fam& F : P->fams) {
+      for (size_t gi = F.g_begin; gi < F.g_end; ++gi) {
+        uint8_t nib[64] = {};
+        if (gk[gi] >= 0) {
+          const uint64_t g0 = g01[2 * gi], g1 = g01[2 * gi + 1];
+          const uint64_t bits = g1 | (g0 << 20) | ((g0 ^ g1) << 40);
+          for (int k = 0; k < 60; ++k) nib[k] = ((bits >> k) & 1u) ? 0x2 : 0xA;  // +1 / -1
+          const int bias = gk[gi] - __builtin_popcount(static_cast<uint32_t>(g0 | g1) & bk::kWin);
+          if (!fp4_sum4(bias, nib + 60)) return fail(BGO_ERR_ARG, "guide bias %d out of range", bias);
+        } else {
+          for (int k = 60; k < 64; ++k) nib[k] = 0xF;  // -6
+        }
+        const size_t tile = gi / 32, col = gi % 32;
+        for (int grp = 0; grp < 2; ++grp) {
+          uint32_t w[4] = {0, 0, 0, 0};
+          for (int i = 0; i < 32; ++i) w[i / 8] |= static_cast<uint32_t>(nib[32 * grp + i]) << (4 * (i % 8));
+          std::memcpy(P->h_btab.data() + (tile * 64 + grp * 32 + col) * 16, w, 16);
+        }
       }
     }
   }

@@ -56,6 +56,40 @@ def algorithmic_ops(L_pos: int, n_strands: int, families) -> tuple:
     return ops, survey
 
 
+ENGINE_NAMES = {1: "batched VALU", 2: "batched int8 MFMA", 3: "batched fp4 MFMA"}
+
+
+def mfma_ops(families) -> int:
+    """Matrix-core ops of the MFMA formulation for one launch: each (guide,
+    PAM-passing window) pair is a 60-long dot product over the tetrahedral
+    code (3 elements per base x 20 bases), i.e. 60 multiply-adds = 120 ops
+    (int8 ops for the int8 engine, fp4 flops for the fp4 engine)."""
+    return sum(120 * g * w for g, w in families)
+
+
+# HBM traffic of one launch of the dominant kernel on the default workload,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the same
+# counters' reading for a copy kernel of known size (profiles/
+# batched_scan_kernel_traffic_hg38_mixed_1024g_4mm.md).  Keyed by
+# (workload, engine); None when not profiled.
+TRAFFIC_BYTES = {}
+
+
+def copy_probe(mbytes: int) -> None:
+    """Calibration for the HBM byte counters: a device copy of `mbytes` MiB
+    (reads and writes exactly that many bytes per launch), three launches.
+    Runs without the scan library in the process."""
+    import torch
+
+    n = mbytes << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {n} bytes")
+
+
 def cpu_baseline(workload, max_mm: int, target_s: float = 12.0) -> dict:
     from oracle.scan import scan_strand
     from beguidedoff_amd.encoding import encode_dna, encode_guide_64bit
@@ -149,6 +183,10 @@ def main() -> int:
     ap.add_argument("--rates", default="2,8,32", help="serving arrival rates (req/s); '' skips")
     ap.add_argument("--serving-duration", type=float, default=15.0,
                     help="seconds of Poisson arrivals per rate")
+    ap.add_argument("--no-engines", action="store_true", help="skip the VALU/MFMA comparison")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device copy of this many MiB (HBM counter calibration; "
+                         "a process of its own: torch carries its own HIP runtime)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,6 +194,10 @@ def main() -> int:
                           "n_gpus": args.gpus}))
         print("bench.py: only --gpus 1 is supported (one MI355X per box)", file=sys.stderr)
         return 2
+
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return 0
 
     from beguidedoff_amd import _lib, synth
     from beguidedoff_amd.batched import DeviceGenome, GuideSpec, Plan, default_capacity
@@ -216,14 +258,26 @@ def main() -> int:
     families = plan.family_stats()
     L_pos = max(0, L - 26)
     ops, survey_ops = algorithmic_ops(L_pos, 2, families)
-    achieved = ops / (kms / 1e3) / 1e12
-    peak = peaks["valu_lane_ops"] / 1e12
+    engine = plan.tuning()["engine"]
+    if engine == 2:  # int8 MFMA compare: priced against the dense int8 MFMA peak
+        bound, unit, dtype = "mfma", "TOP/s", "i8"
+        alg = mfma_ops(families)
+        peak = 5.0e3
+    elif engine == 3:  # fp4 MFMA compare: priced against the dense fp4 MFMA peak
+        bound, unit, dtype = "mfma", "TFLOP/s", "fp4"
+        alg = mfma_ops(families)
+        peak = 10.1e3
+    else:            # VALU compare: 32-bit lane-ops against the VALU issue peak
+        bound, unit, dtype = "valu", "Tops/s", "u32"
+        alg = ops
+        peak = peaks["valu_lane_ops"] / 1e12
+    achieved = alg / (kms / 1e3) / 1e12
     log(f"batched: {ms_per_step:.2f} ms/step (kernel {kms:.2f} ms), {value:.1f} guide·Gbp/s, "
         f"{n} sites, families {families}")
 
     # both compare engines on the same workload (same tiling otherwise)
     engines = {}
-    for eng, name in ((1, "valu"), (2, "mfma")):
+    for eng, name in () if args.no_engines else ((1, "valu"), (2, "mfma_i8"), (3, "mfma_fp4")):
         t_eng = dict(plan.tuning())
         t_eng["engine"] = eng
         p2 = Plan(dg, specs, tuning=t_eng, hit_capacity=default_capacity(len(specs)))
@@ -239,21 +293,25 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "guide·Gbp/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": w.name, "description": w.description, "genome_bp": L,
                    "guides": len(specs), "max_mismatches": args.max_mm,
                    "pams": sorted({s.pam for s in specs}),
-                   "engine": {1: "batched VALU", 2: "batched int8 MFMA"}[plan.tuning()["engine"]],
+                   "engine": ENGINE_NAMES[engine],
                    "graph": not args.eager, "tuning": plan.tuning()},
         "sites": int(n),
         "engines": engines,
         "autotune": tune_trials,
         "kernel_ms": round(kms, 3),
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
-                     "unit": "Tops/s", "frac": round(achieved / peak, 4), "traffic": None,
-                     "ops_per_launch": int(ops), "survey_ops_per_launch": int(survey_ops),
-                     "survey_frac": round(survey_ops / (kms / 1e3) / peaks["valu_lane_ops"], 4),
+        "roofline": {"bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 3),
+                     "unit": unit, "frac": round(achieved / peak, 4),
+                     "traffic": TRAFFIC_BYTES.get((w.name, engine)),
+                     "kernel": "batched_scan_kernel", "ops_per_launch": int(alg),
+                     "pairs_per_launch": int(sum(gg * ww for gg, ww in families)),
+                     "valu_ops_per_launch": int(ops),
+                     "valu_frac": round(ops / (kms / 1e3) / peaks["valu_lane_ops"], 4),
+                     "survey_ops_per_launch": int(survey_ops),
                      "hbm_gbps_planes": round(2 * L / 4 / (kms / 1e3) / 1e9, 1)},
     }
 

@@ -105,7 +105,7 @@ def test_plain_small_config(gpu):
     assert plain_set(w.genome, guides, 3) == oracle_set(w.genome, guides, 3)
 
 
-ENGINES = [1, 2]   # 1: VALU xor/popcount compare, 2: int8 MFMA compare
+ENGINES = [1, 2, 3]   # 1: VALU xor/popcount, 2: int8 MFMA, 3: fp4 MFMA compare
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -227,7 +227,9 @@ def test_tunings_give_identical_sites(gpu):
               {"chunk_positions": 1 << 20, "waves_per_block": 1, "blocks_per_cu": 1},
               {"waves_per_block": 2, "blocks_per_cu": 3, "interleave": 1},
               {"engine": 2, "chunk_positions": 64}, {"engine": 2, "waves_per_block": 1},
-              {"engine": 2, "chunk_positions": 1 << 20, "interleave": 1}):
+              {"engine": 2, "chunk_positions": 1 << 20, "interleave": 1},
+              {"engine": 3, "chunk_positions": 64}, {"engine": 3, "waves_per_block": 1},
+              {"engine": 3, "chunk_positions": 1 << 20, "interleave": 1}):
         r = scan(dg, specs, tuning=t)
         assert hit_set(r.hits) == want, t
     from beguidedoff_amd.autotune import autotune
@@ -253,3 +255,27 @@ def test_mfma_engine_many_guides_high_budget(gpu):
     want = oracle_set(genome, guides, None, mm_per=budgets)
     for engine in ENGINES:
         assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_budget_and_composition_extremes(gpu, engine):
+    """Budgets 0..20 against guides from all-A to no-A bases: the MFMA sign
+    tests see every bias from -20 to +20 (budget - non-A guide bases), and a
+    budget of 20 makes every PAM-passing window a site."""
+    rng = np.random.default_rng(5)
+    contigs, _, _ = build_case(5, length=(2_000, 4_000), n_guides=1)
+    seqs = ["A" * 20, "C" * 20, "T" * 20, "A" * 10 + "G" * 10, "AAAAAAAAAAAAAAAAAAAC",
+            "".join(rng.choice(list("ACGT"), size=20))]
+    guides, budgets = [], []
+    for q in seqs:
+        for k in (0, 1, 4, 10, 19, 20):
+            guides.append((q, "NG"))
+            budgets.append(k)
+    guides.append((contigs[0][1][100:120].replace("N", "A"), "NNN"))
+    budgets.append(0)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
+    want = oracle_set(genome, guides, None, mm_per=budgets)
+    assert len(want) > 1000
+    assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
