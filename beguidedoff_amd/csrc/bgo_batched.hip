@@ -416,30 +416,44 @@ struct WaveState {
 
 // Append this lane's passing windows (bits of pw in block bb) to family f's
 // ring buffer, comparing every 64 buffered windows.  All lanes call this.
+// Each step appends the next window of every lane that has one (slots by
+// ballot + mbcnt), so all lanes write in parallel; the ring always keeps room
+// for one such step.
 template <int ENGINE>
 __device__ inline void append_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
                                      uint32_t s, uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
                                      uint32_t cs, uint32_t lane) {
   uint2* L = w.lists + f * kList;
-  const uint32_t n = __popc(pw);
-  const uint32_t base = atomicAdd(&w.tails[f], n);
-  wave_sync();
-  const uint32_t tail = uniform(w.tails[f]);
+  uint32_t tail = uniform(w.tails[f]);
   uint32_t head = uniform(w.heads[f]);
-  uint32_t k = 0;
-  while (true) {
-    const uint32_t limit = head + kList;
-    while (k < n && base + k < limit) {
+  const uint32_t off0 = (bb << 5) - cs;
+  for (;;) {
+    const uint64_t act = ballot(pw != 0);
+    if (!act) break;
+    if (tail - head > kList - 64) {  // no room for a full step: compare 64
+      wave_sync();
+      if constexpr (ENGINE >= 2)
+        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, 64, cs, lane);
+      else
+        compare_list(a, t, t.fam[f], s, L, head, 64, cs, lane);
+      head += 64;
+      if (lane == 0) w.wstats[f] += 64;
+      wave_sync();
+      continue;
+    }
+    if (pw) {
       const uint32_t j = __builtin_ctz(pw);
       pw &= pw - 1;
-      const uint32_t off = (bb << 5) + j - cs;
+      const uint32_t off = off0 + j;
       const uint32_t W0 = funnel(qb.x, qa.x, j) & kWin;
       const uint32_t W1 = funnel(qb.y, qa.y, j) & kWin;
-      L[(base + k) & (kList - 1)] = make_uint2(W0 | (off << 20), W1 | ((off >> 12) << 20));
-      ++k;
+      L[(tail + lanes_below(act)) & (kList - 1)] =
+          make_uint2(W0 | (off << 20), W1 | ((off >> 12) << 20));
     }
-    wave_sync();
-    if (tail - head < 64) break;
+    tail += static_cast<uint32_t>(__popcll(act));
+  }
+  wave_sync();
+  while (tail - head >= 64) {
     if constexpr (ENGINE >= 2)
       compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, 64, cs, lane);
     else
@@ -448,7 +462,10 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
     if (lane == 0) w.wstats[f] += 64;
     wave_sync();
   }
-  if (lane == 0) w.heads[f] = head;
+  if (lane == 0) {
+    w.heads[f] = head;
+    w.tails[f] = tail;
+  }
   wave_sync();
 }
 
@@ -473,11 +490,19 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
   }
   wave_sync();
   const uint32_t b_first = cs >> 5, b_last = (ce - 1) >> 5;
+  // Plane words of the next 64 blocks load while this step runs.
+  uint32_t nbb = b_first + lane <= b_last ? b_first + lane : b_first;
+  uint2 na = planes[nbb], nb = planes[nbb + 1];
   for (uint32_t b0 = b_first; b0 <= b_last; b0 += 64) {
     const uint32_t b = b0 + lane;
     const bool inb = b <= b_last;
-    const uint32_t bb = inb ? b : b_first;
-    const uint2 qa = planes[bb], qb = planes[bb + 1];
+    const uint32_t bb = nbb;
+    const uint2 qa = na, qb = nb;
+    if (b0 + 64 <= b_last) {
+      nbb = b0 + 64 + lane <= b_last ? b0 + 64 + lane : b_first;
+      na = planes[nbb];
+      nb = planes[nbb + 1];
+    }
     uint32_t vmask = 0;
     if (inb) {
       const uint32_t p0 = bb << 5;
@@ -624,6 +649,7 @@ struct bgo_plan {
   std::vector<uint32_t> h_g01, h_gaux, h_gid, h_gnm;
   std::vector<int32_t> h_gk;
   bgo_tuning tune{};
+  bool engine_auto = false;  // engine chosen per guide set
   // families, grouped into launches of <= kMaxFam
   std::vector<Fam> fams;              // all families (g ranges into the padded table)
   std::vector<int32_t> fam_guides;    // real guides per family
@@ -776,6 +802,7 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   if (r.waves_per_block <= 0) r.waves_per_block = 4;
   if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
     return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");
+  P->engine_auto = r.engine <= 0;
   if (r.engine <= 0) r.engine = 1;
   if (r.engine < 1 || r.engine > 3) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
   r.guide_unroll = r.engine >= 2 ? 32 : bk::kUnroll;
@@ -855,6 +882,12 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     members[it->second].push_back(i);
   }
   const int n_fam = static_cast<int>(members.size());
+  if (P->engine_auto) {
+    // fp4 matrix cores once families average 16 guides (a 32-guide tile is
+    // then at least half used); the VALU compare for sparser sets.
+    P->tune.engine = n_guides >= 16 * n_fam ? 3 : 1;
+    P->tune.guide_unroll = P->tune.engine >= 2 ? 32 : bk::kUnroll;
+  }
   P->fams.clear();
   P->fam_guides.clear();
   P->group_first.clear();
