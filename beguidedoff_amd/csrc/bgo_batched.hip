@@ -47,7 +47,6 @@ namespace bgo {
 namespace bk {
 
 constexpr int kMaxFam = 8;     // PAM families per kernel launch
-constexpr int kList = 128;     // window slots per family per wave
 constexpr int kUnroll = 8;     // guides per compare step
 constexpr int kMaxBlock = 256; // threads per block (4 waves)
 constexpr uint32_t kWin = 0xFFFFFu;
@@ -90,11 +89,22 @@ struct Tab {
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// Per-family window rings of a wave: ring_size(engine) slots, compared in
+// batches of ring_flush(engine) windows (64 = one per lane for the VALU
+// compare and the int8 engine's two A tiles; 128 = four fp4 A tiles, so each
+// guide operand load serves 128 windows).  A slot is {b0 window bits |
+// offset bits 0..11 << 20, b1 window bits | offset bits 12..23 << 20}, the
+// offset counted from the run start.
+__host__ __device__ constexpr uint32_t ring_size(int engine) { return engine == 3 ? 256 : 128; }
+__host__ __device__ constexpr uint32_t ring_flush(int engine) { return engine == 3 ? 128 : 64; }
+
 // Bytes of LDS one wave uses with nf families (see the kernel's layout).
-__host__ __device__ constexpr size_t wave_lds_bytes(int nf) {
-  return static_cast<size_t>(nf) * kList * 8 + kMaxFam * 8 + static_cast<size_t>(nf) * 64 * 4 +
-         2 * kMaxFam * 4;
+__host__ __device__ constexpr size_t wave_lds_bytes(int nf, int engine) {
+  return static_cast<size_t>(nf) * ring_size(engine) * 8 + kMaxFam * 8 +
+         static_cast<size_t>(nf) * 64 * 4 + 2 * kMaxFam * 4;
 }
+
+__device__ inline uint32_t slot_pos(uint2 e) { return (e.x >> 20) | ((e.y >> 20) << 12); }
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -159,15 +169,15 @@ __device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s
   t.hits[idx] = h;
 }
 
-// Compare the first n (<= 64) windows of a family list with all guides of
+// Compare the first n (<= 64) windows of a family ring with all guides of
 // the family.  One window per lane; guides are wave-uniform.
 __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, uint32_t s,
                                     const uint2* L, uint32_t head, uint32_t n, uint32_t cs,
                                     uint32_t lane) {
   if (lane >= n) return;
-  const uint2 e = L[(head + lane) & (kList - 1)];
+  const uint2 e = L[(head + lane) & (ring_size(1) - 1)];
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
-  const uint32_t pos = cs + ((e.x >> 20) | ((e.y >> 20) << 12));
+  const uint32_t pos = cs + slot_pos(e);
   const uint32_t kmax = F.kmax;
   const uint32_t* __restrict__ g01 = t.g01;
   for (uint32_t g = F.g_begin; g < F.g_end; g += kUnroll) {
@@ -271,10 +281,11 @@ __device__ inline uint32_t spread8(uint32_t b) {
 
 // Exact check of the (window, guide) pairs of one guide tile whose
 // accumulator is non-negative: bit r (16 + r) of `bits` = register r of the
-// product with A tile 0 (1).
+// product with A tile abase (abase + 1).
+template <int ENGINE>
 __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const uint2* L,
                                   uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
-                                  uint32_t bits, uint32_t gt) {
+                                  uint32_t bits, uint32_t gt, uint32_t abase) {
   const uint32_t m = lane & 31u, g = lane >> 5;
   while (ballot(bits != 0)) {
     bool cand = false;
@@ -284,13 +295,13 @@ __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const
       bits &= bits - 1;
       const uint32_t r = bit & 15u, tile = bit >> 4;
       const uint32_t row = 8u * (r >> 2) + 4u * g + (r & 3u);
-      const uint32_t widx = 32u * tile + row;
+      const uint32_t widx = 32u * (abase + tile) + row;
       gi = gt + m;
       if (widx < n) {
-        const uint2 e = L[(head + widx) & (kList - 1)];
+        const uint2 e = L[(head + widx) & (ring_size(ENGINE) - 1)];
         W0 = e.x & kWin;
         W1 = e.y & kWin;
-        pos = cs + ((e.x >> 20) | ((e.y >> 20) << 12));
+        pos = cs + slot_pos(e);
         c = __popc((W0 ^ t.g01[2 * gi]) | (W1 ^ t.g01[2 * gi + 1]));
         cand = static_cast<int>(c) <= t.gk[gi];
       }
@@ -299,11 +310,13 @@ __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const
   }
 }
 
-// Sign-bit test of a guide tile's two products (as 32-bit patterns): the
-// common case returns after one AND-reduction and one ballot.
+// Sign-bit test of a guide tile's products with A tiles abase and abase + 1
+// (as 32-bit patterns): the common case returns after one AND-reduction and
+// one ballot.
+template <int ENGINE>
 __device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const uint2* L,
                                   uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
-                                  const v16i& r0, const v16i& r1, uint32_t gt) {
+                                  const v16i& r0, const v16i& r1, uint32_t gt, uint32_t abase) {
   uint32_t all = static_cast<uint32_t>(r0[0]);
 #pragma unroll
   for (int r = 1; r < 16; ++r) all &= static_cast<uint32_t>(r0[r]);
@@ -316,7 +329,7 @@ __device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const
     bits |= static_cast<uint32_t>(r0[r] >= 0) << r;
     bits |= static_cast<uint32_t>(r1[r] >= 0) << (16 + r);
   }
-  mfma_exact(a, t, s, L, head, n, cs, lane, bits, gt);
+  mfma_exact<ENGINE>(a, t, s, L, head, n, cs, lane, bits, gt, abase);
 }
 
 // int8: one 32-guide tile against the (up to) two 32-window A tiles.
@@ -332,62 +345,79 @@ __device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const ui
     acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, zero, 0, 0, 0);
     acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
   }
-  mfma_signs(a, t, s, L, head, n, cs, lane, acc0, acc1, gt);
+  mfma_signs<2>(a, t, s, L, head, n, cs, lane, acc0, acc1, gt, 0);
 }
 
-// fp4: one 32-guide tile (one K = 64 step per A tile).  Scales are 2^0.
-__device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                uint32_t head, uint32_t n, uint32_t cs, uint32_t lane, bool two,
-                                const v8i (&A)[2], uint4 b, uint32_t gt) {
+// fp4 product of one A tile with one guide tile (scales 2^0), as 32-bit
+// patterns for the sign test.
+__device__ inline v16i fp4_mma(const v8i& A, v4i b) {
   const v16f zero = {};
-  const v8i B = {static_cast<int>(b.x), static_cast<int>(b.y), static_cast<int>(b.z),
-                 static_cast<int>(b.w), 0, 0, 0, 0};
-  const v16f p0 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[0], B, zero, 4, 4, 0, 127, 0,
-                                                                  127);
-  v16i r1 = v16i{} - 1;
-  if (two)
-    r1 = __builtin_bit_cast(
-        v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[1], B, zero, 4, 4, 0, 127, 0, 127));
-  mfma_signs(a, t, s, L, head, n, cs, lane, __builtin_bit_cast(v16i, p0), r1, gt);
+  const v8i B = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+  return __builtin_bit_cast(
+      v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, zero, 4, 4, 0, 127, 0, 127));
 }
 
-// Compare the first n (<= 64) windows of a family list with all guides of the
-// family on the matrix cores.  The guide operand table ends with one spare
-// all-zero tile, so the look-ahead load of the next tile stays in bounds.
+// fp4: one 32-guide tile against the nat (1..4) 32-window A tiles in use.
+__device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
+                                uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
+                                uint32_t nat, const v8i (&A)[4], v4i b, uint32_t gt) {
+  v16i p0 = fp4_mma(A[0], b);
+  v16i p1 = v16i{} - 1;
+  if (nat > 1) p1 = fp4_mma(A[1], b);
+  mfma_signs<3>(a, t, s, L, head, n, cs, lane, p0, p1, gt, 0);
+  if (nat > 2) {
+    p0 = fp4_mma(A[2], b);
+    p1 = v16i{} - 1;
+    if (nat > 3) p1 = fp4_mma(A[3], b);
+    mfma_signs<3>(a, t, s, L, head, n, cs, lane, p0, p1, gt, 2);
+  }
+}
+
+// Compare the first n (<= ring_flush(ENGINE)) windows of a family ring with
+// all guides of the family on the matrix cores.  The guide operand tables end
+// with spare all-zero tiles, so the look-ahead loads of later tiles stay in
+// bounds.
 template <int ENGINE>
 __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam& F, uint32_t s,
                                          const uint2* L, uint32_t head, uint32_t n, uint32_t cs,
                                          uint32_t lane) {
+  constexpr uint32_t kAT = ring_flush(ENGINE) / 32;  // A tiles
   const uint32_t m = lane & 31u, g = lane >> 5;
-  const bool two = n > 32;
-  uint32_t lo[2], hi[2];  // code elements 0..31 and 32..63 of window rows m, 32 + m
+  uint32_t lo[kAT], hi[kAT];  // code elements 0..31 and 32..63 of window rows 32 i + m
 #pragma unroll
-  for (int tile = 0; tile < 2; ++tile) {
+  for (uint32_t tile = 0; tile < kAT; ++tile) {
     const uint32_t widx = 32u * tile + m;
     uint2 e = make_uint2(0u, 0u);
-    if (widx < n) e = L[(head + widx) & (kList - 1)];
+    if (widx < n) e = L[(head + widx) & (ring_size(ENGINE) - 1)];
     const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
     lo[tile] = W1 | (W0 << 20);
     hi[tile] = (W0 >> 12) | ((W0 ^ W1) << 8);
   }
   if constexpr (ENGINE == 3) {
-    v8i A[2];
+    v8i A[4];
     const uint32_t pad = g ? 0x22220000u : 0u;  // elements 60..63 = 1.0
 #pragma unroll
-    for (int tile = 0; tile < 2; ++tile) {
+    for (int tile = 0; tile < 4; ++tile) {
       const uint32_t c = g ? hi[tile] : lo[tile];
       A[tile] = v8i{static_cast<int>(spread8(c & 0xFFu)), static_cast<int>(spread8((c >> 8) & 0xFFu)),
                     static_cast<int>(spread8((c >> 16) & 0xFFu)),
                     static_cast<int>(spread8(c >> 24) | pad), 0, 0, 0, 0};
     }
-    const uint4* __restrict__ bp = reinterpret_cast<const uint4*>(t.btab) + (F.g_begin / 32) * 64;
-    uint4 b = bp[lane];
-    for (uint32_t gt = F.g_begin; gt < F.g_end; gt += 32, bp += 64) {
-      const uint4 c = b;
-      b = bp[64 + lane];
-      fp4_tile(a, t, s, L, head, n, cs, lane, two, A, c, gt);
+    const uint32_t nat = (n + 31) / 32;
+    // Two operand registers, each reloaded two tiles ahead (no copies); the
+    // table ends with three spare tiles for these look-ahead loads.
+    const v4i* __restrict__ bt = reinterpret_cast<const v4i*>(t.btab) + (F.g_begin / 32) * 64;
+    const uint32_t n_tiles = (F.g_end - F.g_begin) / 32;
+    v4i b0 = bt[lane], b1 = bt[64 + lane];
+    for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
+      fp4_tile(a, t, s, L, head, n, cs, lane, nat, A, b0, F.g_begin + 32 * ti);
+      b0 = bt[(ti + 2) * 64 + lane];
+      if (ti + 1 < n_tiles)
+        fp4_tile(a, t, s, L, head, n, cs, lane, nat, A, b1, F.g_begin + 32 * ti + 32);
+      b1 = bt[(ti + 3) * 64 + lane];
     }
   } else {
+    const bool two = n > 32;
     v4i A[2][2];
 #pragma unroll
     for (int tile = 0; tile < 2; ++tile) {
@@ -407,37 +437,42 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
 
 // Per-wave LDS state of the scan loop.
 struct WaveState {
-  uint2* lists;                  // [NF][kList] ring buffers of windows
+  uint2* lists;                  // [NF][ring_size] window rings
   uint32_t* pws;                 // [NF][64] pass words of the current blocks
   uint32_t* heads;               // [kMaxFam] consumed ring index
   uint32_t* tails;               // [kMaxFam] appended ring index
   unsigned long long* wstats;    // [kMaxFam] windows compared
+  template <int ENGINE>
+  __device__ uint2* ring(uint32_t f) const {
+    return lists + f * ring_size(ENGINE);
+  }
 };
 
 // Append this lane's passing windows (bits of pw in block bb) to family f's
-// ring buffer, comparing every 64 buffered windows.  All lanes call this.
-// Each step appends the next window of every lane that has one (slots by
-// ballot + mbcnt), so all lanes write in parallel; the ring always keeps room
-// for one such step.
+// ring, comparing every ring_flush(ENGINE) buffered windows.  All lanes call
+// this.  Each step appends the next window of every lane that has one (slots
+// by ballot + mbcnt), so all lanes write in parallel; the ring always keeps
+// room for one such step.
 template <int ENGINE>
 __device__ inline void append_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
                                      uint32_t s, uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
                                      uint32_t cs, uint32_t lane) {
-  uint2* L = w.lists + f * kList;
+  constexpr uint32_t kSize = ring_size(ENGINE), kFlush = ring_flush(ENGINE);
+  uint2* L = w.ring<ENGINE>(f);
   uint32_t tail = uniform(w.tails[f]);
   uint32_t head = uniform(w.heads[f]);
   const uint32_t off0 = (bb << 5) - cs;
   for (;;) {
     const uint64_t act = ballot(pw != 0);
     if (!act) break;
-    if (tail - head > kList - 64) {  // no room for a full step: compare 64
+    if (tail - head > kSize - 64) {  // no room for a full step: compare a batch
       wave_sync();
       if constexpr (ENGINE >= 2)
-        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, 64, cs, lane);
+        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
       else
-        compare_list(a, t, t.fam[f], s, L, head, 64, cs, lane);
-      head += 64;
-      if (lane == 0) w.wstats[f] += 64;
+        compare_list(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
+      head += kFlush;
+      if (lane == 0) w.wstats[f] += kFlush;
       wave_sync();
       continue;
     }
@@ -445,21 +480,20 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
       const uint32_t j = __builtin_ctz(pw);
       pw &= pw - 1;
       const uint32_t off = off0 + j;
-      const uint32_t W0 = funnel(qb.x, qa.x, j) & kWin;
-      const uint32_t W1 = funnel(qb.y, qa.y, j) & kWin;
-      L[(tail + lanes_below(act)) & (kList - 1)] =
-          make_uint2(W0 | (off << 20), W1 | ((off >> 12) << 20));
+      const uint32_t slot = (tail + lanes_below(act)) & (kSize - 1);
+      L[slot] = make_uint2((funnel(qb.x, qa.x, j) & kWin) | (off << 20),
+                           (funnel(qb.y, qa.y, j) & kWin) | ((off >> 12) << 20));
     }
     tail += static_cast<uint32_t>(__popcll(act));
   }
   wave_sync();
-  while (tail - head >= 64) {
+  while (tail - head >= kFlush) {
     if constexpr (ENGINE >= 2)
-      compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, 64, cs, lane);
+      compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
     else
-      compare_list(a, t, t.fam[f], s, L, head, 64, cs, lane);
-    head += 64;
-    if (lane == 0) w.wstats[f] += 64;
+      compare_list(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
+    head += kFlush;
+    if (lane == 0) w.wstats[f] += kFlush;
     wave_sync();
   }
   if (lane == 0) {
@@ -472,8 +506,8 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
 // Stream positions [cs, ce) of strand s.  Each lane owns a 32-position block:
 // the PAM test of every family runs on whole plane words (one allowed-set op
 // per PAM position for 32 positions), then the passing windows are appended
-// to the family's ring buffer in LDS; every 64 buffered windows are compared
-// against the family's guides (one window per lane).
+// to the family's ring in LDS; every ring_flush(ENGINE) buffered windows are
+// compared against the family's guides.
 template <int NF, int ENGINE>
 __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w, uint32_t s,
                                 uint32_t cs, uint32_t ce, uint32_t lane) {
@@ -540,9 +574,9 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
     const uint32_t c = uniform(w.tails[f]) - head;
     if (c) {
       if constexpr (ENGINE >= 2)
-        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
+        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cs, lane);
       else
-        compare_list(a, t, t.fam[f], s, w.lists + f * kList, head, c, cs, lane);
+        compare_list(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cs, lane);
       if (lane == 0) w.wstats[f] += c;
     }
   }
@@ -562,13 +596,14 @@ __global__ __launch_bounds__(kMaxBlock) void batched_scan_kernel(
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t n_waves = blockDim.x >> 6;
-  // LDS per wave: NF*kList windows (8 B), kMaxFam stats (8 B), NF*64 pass
-  // words, kMaxFam heads and tails (4 B each).
-  char* my = reinterpret_cast<char*>(lds) + wave * wave_lds_bytes(NF);
+  // LDS per wave: NF window rings (ring_size x 8 B), kMaxFam stats (8 B),
+  // NF*64 pass words, kMaxFam heads and tails (4 B each).
+  constexpr size_t kRings = NF * ring_size(ENGINE) * 8;
+  char* my = reinterpret_cast<char*>(lds) + wave * wave_lds_bytes(NF, ENGINE);
   WaveState w;
   w.lists = reinterpret_cast<uint2*>(my);
-  w.wstats = reinterpret_cast<unsigned long long*>(my + NF * kList * 8);
-  w.pws = reinterpret_cast<uint32_t*>(my + NF * kList * 8 + kMaxFam * 8);
+  w.wstats = reinterpret_cast<unsigned long long*>(my + kRings);
+  w.pws = reinterpret_cast<uint32_t*>(my + kRings + kMaxFam * 8);
   w.heads = w.pws + NF * 64;
   w.tails = w.heads + kMaxFam;
   if (lane < kMaxFam) w.wstats[lane] = 0;
@@ -819,7 +854,7 @@ int plan_configure_launch(bgo_plan* P) {
   int min_occ = 1 << 30;
   for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
     const int nf = std::min<int>(bk::kMaxFam, static_cast<int>(P->fams.size()) - P->group_first[gi]);
-    const size_t per_wave = bk::wave_lds_bytes(nf);
+    const size_t per_wave = bk::wave_lds_bytes(nf, P->tune.engine);
     const size_t lds = per_wave * P->tune.waves_per_block;
     int occ = 0;
     BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bk::kernel_for(nf, P->tune.engine), P->threads,
@@ -943,8 +978,8 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   const size_t n_pad = gk.size();
   P->n_guides = n_guides;
 
-  // MFMA guide operands (each table ends with one spare all-zero tile for the
-  // kernel's look-ahead load).
+  // MFMA guide operands (the tables end with spare all-zero tiles for the
+  // kernel's look-ahead loads: one for int8, three for fp4).
   //  * int8: per 32-guide tile, per k-half j and lane l (column l % 32, lane
   //    group l / 32) the 16 int8 of k = 32j + 16(l/32) .. +15 of the guide's
   //    tetrahedral code; k 60..63 hold the family's kmax - 15; padding
@@ -976,7 +1011,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       }
     }
   } else if (P->tune.engine == 3) {
-    P->h_btab.assign((n_pad / 32 + 1) * 64 * 16, 0);
+    P->h_btab.assign((n_pad / 32 + 3) * 64 * 16, 0);
     for (const Fam& F : P->fams) {
       for (size_t gi = F.g_begin; gi < F.g_end; ++gi) {
         uint8_t nib[64] = {};
