@@ -308,6 +308,10 @@ __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const
     }
     if (ballot(cand)) record_candidates(a, t, s, gi, pos, W0, W1, c, cand, lane);
   }
+  // Leave no memory operation of this rare path outstanding: the common path
+  // keeps a guide operand load in flight across this join, and a pending
+  // load here would make the compiler wait for everything at the join.
+  __builtin_amdgcn_s_waitcnt(0);
 }
 
 // Sign-bit test of a guide tile's products with A tiles abase and abase + 1
