@@ -14,16 +14,18 @@
 // uint32 words of the reference's little-endian bytes.
 //
 // Work decomposition.  Persistent waves pull work items (a strand and a run
-// of `chunk` positions) from an atomic counter.  A wave streams its run 64
-// positions at a time (one position per lane): it extracts the 26-base
-// window+PAM of both planes, evaluates the IUPAC PAM of every PAM family of
-// the launch with three bit-selects, and appends the passing windows to a
-// per-family list in LDS (ballot + mbcnt compaction, no atomics).  Whenever a
-// family list holds 64 windows, the wave compares them against all guides of
-// that family (guides are wave-uniform, read through the scalar cache):
-// 8 guides per step, min-reduced so the common "no site" case costs one
-// compare per 8 guides.  Candidates take the exact path: N correction,
-// per-guide budget, epilogue, and a wave-aggregated append of hit records.
+// of `chunk` positions) from an atomic counter.  Each lane of a wave owns a
+// 32-position block: from the plane words it tests the IUPAC PAM of every PAM
+// family of the launch for all 32 positions at once (one bit-select per PAM
+// base), and the passing windows are appended to per-family rings in LDS
+// (every lane writes its next window in each step; slots by ballot + mbcnt).
+// Whenever a ring holds a batch, the wave compares the batch against all
+// guides of that family with one of three engines: VALU xor/bitop3/popcount
+// (8 wave-uniform guides per scalar load, min-reduced), or the matrix cores
+// over a tetrahedral +-1 code of the bases (int8 or fp4 MFMA, sign-bit
+// reduction).  Candidates take the exact path: N correction, per-guide
+// budget, fused epilogue and a wave-aggregated append of hit records.
+
 #include <algorithm>
 #include <cstring>
 #include <map>

@@ -70,9 +70,9 @@ def mfma_ops(families) -> int:
 # HBM traffic of one launch of the dominant kernel on the default workload,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the same
 # counters' reading for a copy kernel of known size (profiles/
-# batched_scan_kernel_traffic_hg38_mixed_1024g_4mm.md).  Keyed by
-# (workload, engine); None when not profiled.
-TRAFFIC_BYTES = {}
+# batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md).  Keyed by (workload,
+# engine); None when not profiled.
+TRAFFIC_BYTES = {("hg38_mixed", 3): 1_605_344_120}
 
 
 def copy_probe(mbytes: int) -> None:
@@ -309,8 +309,6 @@ def main() -> int:
                      "traffic": TRAFFIC_BYTES.get((w.name, engine)),
                      "kernel": "batched_scan_kernel", "ops_per_launch": int(alg),
                      "pairs_per_launch": int(sum(gg * ww for gg, ww in families)),
-                     "valu_ops_per_launch": int(ops),
-                     "valu_frac": round(ops / (kms / 1e3) / peaks["valu_lane_ops"], 4),
                      "survey_ops_per_launch": int(survey_ops),
                      "hbm_gbps_planes": round(2 * L / 4 / (kms / 1e3) / 1e9, 1)},
     }
