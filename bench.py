@@ -10,6 +10,9 @@ persistent kernel (hipGraph replay) including the copy of all site records
 to the host; the genome and guide table are resident in HBM before timing.
 The plain drop-in kernel (the reference's decomposition) is timed on the
 same workload, and the oracle (CPU restatement) on a bounded sample of it.
+Beside them, the reference's CPU fast mode (exact anchors at the CCTOP cutoff
+5, restated in C over its own index) runs on the whole genome for the 20-nt
+guides, and its keep/remove verdicts are checked against the GPU sites.
 
 Prints one JSON line.  guide*Gbp/s = guides x single-strand genome Gbp /
 seconds, both strands always scanned.
@@ -121,6 +124,82 @@ def cpu_baseline(workload, max_mm: int, target_s: float = 12.0) -> dict:
     return {"value": value, "unit": "guide·Gbp/s", "cores": threads, "kind": "port",
             "sample": f"oracle scan of all {len(guides)} guides, both strands, {n_pos} positions "
                       f"({dt:.1f} s, {threads} threads)"}
+
+
+def fast_mode_leg(workload, hits: np.ndarray, threads: int, cutoff: float = 5.0) -> dict:
+    """The reference's CPU fast mode (exact cold-region anchors at the CCTOP
+    cutoff), restated in C over its own index of the forward genome: index
+    build, query time single-threaded and on all threads, and its keep/remove
+    verdict per 20-nt guide against this run's GPU sites (remove = an
+    editable, contig-internal site with score <= cutoff)."""
+    from oracle.fastmode import FastIndex, any_editable_row
+
+    g = workload.genome
+    sel = [i for i, x in enumerate(workload.guides) if len(x.guide_seq) == 20]
+    guides = [(workload.guides[i].guide_seq, workload.guides[i].pam) for i in sel]
+    ix = FastIndex(g, threads=threads)
+    build_s = ix.build_s
+    t0 = time.perf_counter()
+    res = ix.search(guides, cutoff, threads=1)
+    q1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = ix.search(guides, cutoff, threads=threads)
+    qn = time.perf_counter() - t0
+    index_bytes = ix.nbytes
+    ix.close()
+
+    offs = np.asarray(g["chrom_offsets"], dtype=np.int64)
+    lens = np.asarray(g["chrom_lengths"], dtype=np.int64)
+    nm = g.get("n_mask_le_fwd")
+
+    def fwd_window(strand, pos, win):
+        k = int(np.searchsorted(offs, pos, side="right") - 1)
+        off, ln = int(offs[k]), int(lens[k])
+        if pos + win > off + ln:
+            return None  # junction window (scanned by the extensive mode only)
+        return pos if strand == 0 else off + ln - (pos - off) - win
+
+    def has_n(left, win):
+        if nm is None:
+            return False
+        return any((nm[i >> 3] >> (i & 7)) & 1 for i in range(left, left + win))
+
+    remove_ext, n_ext = {}, {}
+    gsel = set(sel)
+    keep = hits[(hits["editable"] == 1) & (hits["score"] <= cutoff)]
+    for h in keep:
+        gi = int(h["guide"])
+        if gi not in gsel:
+            continue
+        win = 20 + len(workload.guides[gi].pam)
+        left = fwd_window(int(h["strand"]), int(h["pos"]), win)
+        if left is None:
+            continue
+        remove_ext[gi] = True
+        n_ext[gi] = n_ext.get(gi, False) or has_n(left, win)
+    agree = disagree = disagree_n = overflow = 0
+    n_remove_fast = 0
+    for gi, r in zip(sel, res):
+        x = workload.guides[gi]
+        rf = any_editable_row(r, x.editor_type, x.edit_window_start, x.edit_window_end)
+        n_remove_fast += rf
+        overflow += r.overflow
+        if rf == remove_ext.get(gi, False):
+            agree += 1
+        else:
+            disagree += 1
+            win = 20 + len(x.pam)
+            if n_ext.get(gi, False) or any(has_n(int(l), win) for l in r.leftmost.tolist()):
+                disagree_n += 1
+    gbp = g["total_bp"] / 1e9
+    return {"value": round(len(guides) * gbp / qn, 1), "value_1t": round(len(guides) * gbp / q1, 1),
+            "unit": "guide·Gbp/s", "cores": threads, "kind": "port",
+            "sample": f"all {len(guides)} 20-nt guides, whole genome, cutoff {cutoff}",
+            "query_s": round(qn, 4), "query_s_1t": round(q1, 4), "index_build_s": round(build_s, 2),
+            "index_bytes": index_bytes,
+            "verdicts": {"guides": len(guides), "agree": agree, "disagree": disagree,
+                         "disagree_at_n_windows": disagree_n, "remove_fast": n_remove_fast,
+                         "remove_extensive": len(remove_ext), "anchor_cap_overflow": overflow}}
 
 
 def serving_leg(dg, pool, max_mm: int, rates, duration: float, seed: int = 0) -> dict:
@@ -352,6 +431,10 @@ def main() -> int:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(w, args.max_mm)
         log(f"cpu baseline {out['cpu_baseline']} in {time.perf_counter() - t0:.1f} s")
+        t0 = time.perf_counter()
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        out["cpu_fast_mode"] = fast_mode_leg(w, hits, max(1, min(threads, 64)))
+        log(f"cpu fast mode {out['cpu_fast_mode']} in {time.perf_counter() - t0:.1f} s")
 
     plan.close()
     dg.close()

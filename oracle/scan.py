@@ -19,8 +19,12 @@ LIB = HERE / "libbgo_oracle.so"
 _lib = None
 
 
+SOURCES = ("bgo_oracle.c", "bgo_fast.c")
+
+
 def build(force: bool = False) -> Path:
-    if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "bgo_oracle.c").stat().st_mtime:
+    newest = max((HERE / s).stat().st_mtime for s in SOURCES)
+    if force or not LIB.exists() or LIB.stat().st_mtime < newest:
         subprocess.run(["make", "-s", "-C", str(HERE), "-B" if force else "libbgo_oracle.so"],
                        check=True)
     return LIB
