@@ -149,11 +149,14 @@ class Plan:
             raise RuntimeError("plan released")
         return self._h
 
-    def set_guides(self, specs: Sequence[GuideSpec]) -> None:
-        """Swap the guide set (waits for the previous launch)."""
+    def set_guides(self, specs: Sequence[GuideSpec], descs: Optional[np.ndarray] = None) -> None:
+        """Swap the guide set (waits for the previous launch).  `descs` may pass
+        the specs' descriptors when the caller already has them."""
         self.specs = list(specs)
         self.n_guides = len(self.specs)
-        self.descs = guide_descs(self.specs)
+        self.descs = guide_descs(self.specs) if descs is None else np.ascontiguousarray(descs)
+        if self.descs.size != self.n_guides:
+            raise ValueError("descriptor count differs from the guide count")
         _lib.check(_lib.load().bgo_plan_set_guides(self.handle, _lib.ptr(self.descs),
                                                    self.n_guides))
 

@@ -32,7 +32,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from .batched import HIT_DTYPE, DeviceGenome, GuideSpec, Plan
+from .batched import HIT_DTYPE, DeviceGenome, GuideSpec, Plan, guide_descs
 from .pam import IUPAC_MASK
 
 
@@ -58,6 +58,7 @@ class Request:
     specs: List[GuideSpec]
     arrival: float
     reserve: int
+    descs: np.ndarray = None
     launched: int = 0
     retired: int = 0
     admitted_at: float = 0.0
@@ -93,6 +94,7 @@ class ScanServer:
                        for k in range(n_segments)]
         self._plans = [Plan(genome, [], tuning=tuning, hit_capacity=1 << 16)
                        for _ in range(pipeline)]
+        self._plan_set = {id(p): () for p in self._plans}  # request ids a plan's tables hold
         self._free = collections.deque(self._plans)
         self._queue: collections.deque = collections.deque()
         self._active: List[Request] = []
@@ -112,11 +114,13 @@ class ScanServer:
 
     def submit(self, specs: Sequence[GuideSpec]) -> Request:
         specs = list(specs)
+        arrival = time.perf_counter()
+        descs = guide_descs(specs)  # validates the guides on the caller's thread
         reserve = max(self.min_reserve, int(4 * estimate_hits(specs, self.genome.total_bp)) + 64)
         with self._cv:
             if self._stop:
                 raise RuntimeError("server stopped")
-            req = Request(self._rid, specs, time.perf_counter(), min(reserve, self.hit_budget))
+            req = Request(self._rid, specs, arrival, min(reserve, self.hit_budget), descs)
             self._rid += 1
             self._queue.append(req)
             self._cv.notify_all()
@@ -184,7 +188,10 @@ class ScanServer:
         cap = max(1 << 12, int(2 * sum(r.reserve for r in reqs) / self.n_segments))
         if cap > plan.hit_capacity:
             plan.set_hit_capacity(cap)
-        plan.set_guides(specs)
+        key = tuple(r.rid for r in reqs)
+        if self._plan_set[id(plan)] != key:  # the active set changed: new guide tables
+            plan.set_guides(specs, np.concatenate([r.descs for r in reqs]))
+            self._plan_set[id(plan)] = key
         b, e = self.bounds[seg]
         plan.launch(b, e, 3)
         self.launches += 1
