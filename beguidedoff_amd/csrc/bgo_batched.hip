@@ -54,7 +54,7 @@ constexpr int kMaxBlock = 256; // threads per block (4 waves)
 constexpr uint32_t kWin = 0xFFFFFu;
 
 struct Fam {
-  uint32_t nibs;            // allowed-base nibble of PAM position i at bits 4i..4i+3
+  uint32_t nibs;            // allowed-base nibble of PAM position i at bits 4i..4i+3 (N past the PAM)
   uint32_t plen;            // PAM length 1..6
   uint32_t g_begin, g_end;  // padded guide range
   uint32_t kmax;            // largest budget among the family's guides
@@ -210,26 +210,15 @@ __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, u
 }
 
 // Bases of a PAM position whose 2-bit code (x1 x0) is in the allowed set
-// `nib` (bit b = code b allowed), for 32 positions at once.
+// `nib` (bit b = code b allowed), for 32 positions at once.  nib is
+// wave-uniform; the select x1 ? (x0 ? n3 : n2) : (x0 ? n1 : n0) over its bit
+// masks needs no branches.
 __device__ inline uint32_t allowed(uint32_t nib, uint32_t x0, uint32_t x1) {
-  switch (nib) {
-    case 0x1: return ~(x0 | x1);      // A
-    case 0x2: return x0 & ~x1;        // C
-    case 0x3: return ~x1;             // M = A|C
-    case 0x4: return ~x0 & x1;        // G
-    case 0x5: return ~x0;             // R = A|G
-    case 0x6: return x0 ^ x1;         // S = C|G
-    case 0x7: return ~(x0 & x1);      // V = not T
-    case 0x8: return x0 & x1;         // T
-    case 0x9: return ~(x0 ^ x1);      // W = A|T
-    case 0xA: return x0;              // Y = C|T
-    case 0xB: return x0 | ~x1;        // H = not G
-    case 0xC: return x1;              // K = G|T
-    case 0xD: return ~x0 | x1;        // D = not C
-    case 0xE: return x0 | x1;         // B = not A
-    case 0xF: return ~0u;             // N
-    default: return 0u;
-  }
+  const uint32_t n0 = 0u - (nib & 1u), n1 = 0u - ((nib >> 1) & 1u);
+  const uint32_t n2 = 0u - ((nib >> 2) & 1u), n3 = 0u - ((nib >> 3) & 1u);
+  const uint32_t lo = n0 ^ (x0 & (n0 ^ n1));
+  const uint32_t hi = n2 ^ (x0 & (n2 ^ n3));
+  return (x1 & hi) | (~x1 & lo);
 }
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -518,11 +507,10 @@ template <int NF, int ENGINE>
 __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w, uint32_t s,
                                 uint32_t cs, uint32_t ce, uint32_t lane) {
   const uint2* __restrict__ planes = s ? a.planes1 : a.planes0;
-  uint32_t nibs[NF], plen[NF];
+  uint32_t nibs[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     nibs[f] = t.fam[f].nibs;
-    plen[f] = t.fam[f].plen;
   }
   if (lane < kMaxFam) {
     w.tails[lane] = 0;
@@ -530,19 +518,28 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
   }
   wave_sync();
   const uint32_t b_first = cs >> 5, b_last = (ce - 1) >> 5;
-  // Plane words of the next 64 blocks load while this step runs.
-  uint32_t nbb = b_first + lane <= b_last ? b_first + lane : b_first;
-  uint2 na = planes[nbb], nb = planes[nbb + 1];
+  // Each lane loads one plane word per step, of block min(b, b_last + 1) (the
+  // plane array extends past every scanned block).  The following block's
+  // word comes from lane + 1 (DPP wave_shl:1); lane 63 takes it from a
+  // wave-uniform (scalar) load.  Both load two steps ahead.
+  const uint32_t b_end = b_last + 1;
+  uint2 w1 = planes[min(b_first + lane, b_end)], w2 = planes[min(b_first + 64 + lane, b_end)];
+  uint2 e1 = planes[min(b_first + 64, b_end)], e2 = planes[min(b_first + 128, b_end)];
   for (uint32_t b0 = b_first; b0 <= b_last; b0 += 64) {
-    const uint32_t b = b0 + lane;
-    const bool inb = b <= b_last;
-    const uint32_t bb = nbb;
-    const uint2 qa = na, qb = nb;
-    if (b0 + 64 <= b_last) {
-      nbb = b0 + 64 + lane <= b_last ? b0 + 64 + lane : b_first;
-      na = planes[nbb];
-      nb = planes[nbb + 1];
-    }
+    const uint32_t bb = b0 + lane;
+    const bool inb = bb <= b_last;
+    const uint2 qa = w1, e = e1;
+    w1 = w2;
+    w2 = planes[min(b0 + 128 + lane, b_end)];
+    e1 = e2;
+    e2 = planes[min(b0 + 192, b_end)];
+    const uint2 qb = make_uint2(
+        static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(e.x),
+                                                          static_cast<int>(qa.x), 0x130, 0xF,
+                                                          0xF, false)),
+        static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(e.y),
+                                                          static_cast<int>(qa.y), 0x130, 0xF,
+                                                          0xF, false)));
     uint32_t vmask = 0;
     if (inb) {
       const uint32_t p0 = bb << 5;
@@ -560,10 +557,15 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       uint32_t pw = vmask;
+      // The family's nibbles stay in one SGPR; their masks are rederived
+      // here (a few scalar ops) rather than hoisted out of the loop, which
+      // would take 4 SGPRs per PAM base.
+      uint32_t fn = nibs[f];
+      asm volatile("" : "+s"(fn));
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
-        const uint32_t nib = (nibs[f] >> (4 * i)) & 0xFu;
-        if (i < static_cast<int>(plen[f]) && nib != 0xFu) pw &= allowed(nib, f0[i], f1[i]);
+        const uint32_t nib = (fn >> (4 * i)) & 0xFu;
+        if (nib != 0xFu) pw &= allowed(nib, f0[i], f1[i]);
       }
       w.pws[f * 64 + lane] = pw;
     }
@@ -816,7 +818,8 @@ uint64_t fam_key(const bgo_guide_desc& d) {
 
 Fam make_fam(const bgo_guide_desc& d) {
   Fam F{};
-  for (int i = 0; i < d.pam_len; ++i) F.nibs |= (d.pam_mask[i] & 0xFu) << (4 * i);
+  // positions past the PAM read as N (no constraint)
+  for (int i = 0; i < 6; ++i) F.nibs |= (i < d.pam_len ? d.pam_mask[i] & 0xFu : 0xFu) << (4 * i);
   F.plen = d.pam_len;
   return F;
 }

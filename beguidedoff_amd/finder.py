@@ -571,21 +571,37 @@ def assemble_results(finder: "beguidedoff", guides: List[GuideInput],
             if idx.size:
                 per_id[g.guide_id].append((key_index[sg], input_index[id(g)], sg, g, idx))
 
-    frames = []
-    cache: Dict[str, Optional[pd.DataFrame]] = {}
+    blocks = []
+    cache: Dict[str, Optional[Dict[str, np.ndarray]]] = {}
     for g in guides:
         gid = g.guide_id
         if gid not in cache:
-            cache[gid] = _id_frame(finder, decoded, per_id.get(gid, []), below.get(gid, 0),
-                                   return_all, chrom_rank, names, offs)
+            cache[gid] = _id_columns(finder, decoded, per_id.get(gid, []), below.get(gid, 0),
+                                     return_all, chrom_rank, names, offs)
         if cache[gid] is not None:
-            frames.append(cache[gid])
-    if not frames:
+            blocks.append(cache[gid])
+    if not blocks:
         return pd.DataFrame()
-    return pd.concat(frames, ignore_index=True)
+    # one table for all guides: the per-guide blocks in input order
+    return pd.DataFrame({c: np.concatenate([b[c] for b in blocks]) for c in OUTPUT_COLUMNS},
+                        columns=OUTPUT_COLUMNS)
 
 
-def _id_frame(finder, decoded, parts, n_below, return_all, chrom_rank, names, offs):
+_ROUND_CACHE: Dict[float, float] = {}
+
+
+def _round6(x: float) -> float:
+    """Python's round(x, 6) (the reference's rounding), memoised per score."""
+    r = _ROUND_CACHE.get(x)
+    if r is None:
+        r = round(x, 6)
+        _ROUND_CACHE[x] = r
+    return r
+
+
+def _id_columns(finder, decoded, parts, n_below, return_all, chrom_rank, names, offs):
+    """Output columns of one guide id: its hits sorted by (score, chromosome
+    name, position, strand, scan key, input order), top-N unless return_all."""
     if not parts:
         return None
     cols = defaultdict(list)
@@ -611,7 +627,7 @@ def _id_frame(finder, decoded, parts, n_below, return_all, chrom_rank, names, of
         order = order[: finder.n_top_offtargets]
     if order.size == 0:
         return None
-    return pd.DataFrame({
+    return {
         "guide_rna_sequence": c["gseq"][order],
         "base_editor": c["editor"][order],
         "ot_rank": np.arange(1, order.size + 1, dtype=np.int64),
@@ -623,6 +639,6 @@ def _id_frame(finder, decoded, parts, n_below, return_all, chrom_rank, names, of
         "n_mismatches": c["n_mm"][order].astype(np.int64),
         "mismatch_positions": np.array([_mm_string(int(m)) for m in c["mm"][order]],
                                        dtype=object),
-        "score": np.array([round(float(x), 6) for x in c["score"][order]], dtype=np.float64),
+        "score": np.array([_round6(float(x)) for x in c["score"][order]], dtype=np.float64),
         "n_offtargets_below_threshold": np.full(order.size, n_below, dtype=np.int64),
-    }, columns=OUTPUT_COLUMNS)
+    }
