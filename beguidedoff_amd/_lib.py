@@ -61,6 +61,8 @@ _SIGNATURES = {
     "bgo_plan_launch": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int32]),
     "bgo_plan_wait": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_float)]),
     "bgo_plan_fetch": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "bgo_plan_fetch_pinned": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_int64)]),
     "bgo_plan_graph_launch": (C.c_int, [_P, C.c_int32]),
     "bgo_plan_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "bgo_plan_family_stats": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P, _P]),

@@ -185,6 +185,19 @@ class Plan:
                                               C.byref(got)))
         return out[: got.value]
 
+    def fetch_view(self, n: Optional[int] = None) -> np.ndarray:
+        """Like fetch, but the records land in the plan's page-locked buffer by
+        direct DMA and the array is a view of it: valid until the plan's next
+        fetch_view or close (copy it to keep it longer)."""
+        ptr = C.c_void_p()
+        got = C.c_int64(0)
+        _lib.check(_lib.load().bgo_plan_fetch_pinned(self.handle, -1 if n is None else int(n),
+                                                     C.byref(ptr), C.byref(got)))
+        if got.value == 0:
+            return np.zeros(0, dtype=HIT_DTYPE)
+        buf = (C.c_char * (got.value * HIT_DTYPE.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=HIT_DTYPE, count=got.value)
+
     def tuning(self) -> dict:
         t = _lib.Tuning()
         _lib.check(_lib.load().bgo_plan_get_tuning(self.handle, C.byref(t)))

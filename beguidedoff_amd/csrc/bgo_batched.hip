@@ -711,6 +711,8 @@ struct bgo_plan {
   std::vector<int8_t> h_btab;
   uint32_t n_counter_words = 0;
   bgo_hit* d_hits = nullptr;
+  bgo_hit* h_pinned = nullptr;  // page-locked copy of the records (fetch_pinned)
+  int64_t pinned_cap = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -886,6 +888,7 @@ extern "C" int bgo_plan_destroy(bgo_plan* P) {
   if (P->graph) (void)hipGraphExecDestroy(P->graph);
   if (P->d_mem) (void)hipFree(P->d_mem);
   if (P->d_hits) (void)hipFree(P->d_hits);
+  if (P->h_pinned) (void)hipHostFree(P->h_pinned);
   if (P->ev0) (void)hipEventDestroy(P->ev0);
   if (P->ev1) (void)hipEventDestroy(P->ev1);
   if (P->stream) (void)hipStreamDestroy(P->stream);
@@ -1281,6 +1284,38 @@ extern "C" int bgo_plan_fetch(bgo_plan* P, bgo_hit* out, int64_t cap, int64_t* n
     BGO_HIP(hipStreamSynchronize(P->stream));
   }
   *n_copied = std::max<int64_t>(m, 0);
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_fetch_pinned(bgo_plan* P, int64_t cap, const bgo_hit** out,
+                                     int64_t* n_copied) {
+  if (!P || !out || !n_copied) return fail(BGO_ERR_ARG, "bad arguments");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  unsigned long long n = 0;
+  BGO_HIP(hipMemcpyAsync(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost, P->stream));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  int64_t m = std::min<int64_t>(static_cast<int64_t>(n), P->hit_cap);
+  if (cap >= 0) m = std::min(m, cap);
+  m = std::max<int64_t>(m, 0);
+  if (m > P->pinned_cap) {
+    const int64_t want = std::max<int64_t>(m, P->pinned_cap * 3 / 2);
+    if (P->h_pinned) (void)hipHostFree(P->h_pinned);
+    P->h_pinned = nullptr;
+    P->pinned_cap = 0;
+    BGO_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->h_pinned), want * sizeof(bgo_hit),
+                          hipHostMallocDefault));
+    P->pinned_cap = want;
+  }
+  if (m > 0) {
+    BGO_HIP(hipMemcpyAsync(P->h_pinned, P->d_hits, m * sizeof(bgo_hit), hipMemcpyDeviceToHost,
+                           P->stream));
+    BGO_HIP(hipStreamSynchronize(P->stream));
+  }
+  *out = P->h_pinned;
+  *n_copied = m;
   return BGO_OK;
 }
 

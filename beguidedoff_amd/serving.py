@@ -207,10 +207,10 @@ class ScanServer:
             n, ms = plan.wait()
             self.reruns += 1
         self.busy_s += ms / 1e3
-        hits = plan.fetch(n)
-        self._free.append(plan)
+        hits = plan.fetch_view(n)
         order = np.argsort(hits["guide"], kind="stable")
-        hits = hits[order]
+        hits = hits[order]  # a copy: the plan's buffer may be refilled from here on
+        self._free.append(plan)
         g = hits["guide"]
         now = time.perf_counter()
         finished = []

@@ -318,7 +318,7 @@ def main() -> int:
         n, ms = plan.wait()
         if n > plan.hit_capacity:
             raise RuntimeError(f"hit buffer too small ({n} > {plan.hit_capacity})")
-        hits = plan.fetch(n)
+        hits = plan.fetch_view(n)  # all site records, DMA'd into page-locked host memory
         return n, ms, hits
 
     for i in range(args.warmup):
@@ -330,6 +330,7 @@ def main() -> int:
         n, ms, hits = step()
         kernel_ms.append(ms)
     wall = time.perf_counter() - t0
+    hits = hits.copy()  # the last step's records, kept past the plan's next fetch
     ms_per_step = wall * 1e3 / args.steps
     gbp = L / 1e9
     value = len(specs) * gbp / (ms_per_step / 1e3)

@@ -205,6 +205,14 @@ int bgo_plan_wait(bgo_plan* p, int64_t* n_hits_out, float* kernel_ms_out);
 /* Copy min(n_hits, capacity, cap) records of the last launch to host memory. */
 int bgo_plan_fetch(bgo_plan* p, bgo_hit* out, int64_t cap, int64_t* n_copied);
 
+/* The first min(cap, sites, capacity) records (cap < 0: no limit), copied into
+ * a page-locked buffer the plan owns (grown as needed; direct DMA, no staging
+ * copy).  *out points to it and stays valid until the plan's next
+ * fetch_pinned call or its destruction.  No counterpart in the reference: the
+ * zero-copy fetch serves loops that consume the records right away (serving,
+ * benchmark steps). */
+int bgo_plan_fetch_pinned(bgo_plan* p, int64_t cap, const bgo_hit** out, int64_t* n_copied);
+
 /* Capture [counter reset + scan of both strands over the full genome] in a
  * hipGraph once, then replay it with bgo_plan_graph_launch (same wait/fetch).
  * No memset nodes are recorded: counters are cleared by a kernel. */

@@ -279,3 +279,20 @@ def test_budget_and_composition_extremes(gpu, engine):
     want = oracle_set(genome, guides, None, mm_per=budgets)
     assert len(want) > 1000
     assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
+
+
+def test_fetch_view_equals_fetch(gpu):
+    """The page-locked fetch returns the same records as the copying fetch,
+    also after the buffer grows and when capped."""
+    contigs, guides, max_mm = build_case(21, length=(40_000, 60_000), n_guides=6)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    p = Plan(dg, [GuideSpec(s, pm, 6) for s, pm in guides], hit_capacity=1 << 20)
+    p.launch()
+    n, _ = p.wait()
+    assert n > 10
+    a = p.fetch(n)
+    for cap in (7, None):
+        v = p.fetch_view(cap)
+        assert np.array_equal(v, a[: v.size]) and v.size == (7 if cap else n)
+    p.close()
