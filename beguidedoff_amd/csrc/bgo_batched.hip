@@ -352,6 +352,15 @@ __device__ inline v16i fp4_mma(const v8i& A, v4i b) {
       v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, zero, 4, 4, 0, 127, 0, 127));
 }
 
+// AND of a product's 16 elements as 32-bit patterns: the sign bit is clear
+// iff some element is >= 0.
+__device__ inline uint32_t and16(const v16i& p) {
+  uint32_t x = static_cast<uint32_t>(p[0]);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) x &= static_cast<uint32_t>(p[r]);
+  return x;
+}
+
 // fp4: one 32-guide tile against the nat (1..4) 32-window A tiles in use.
 __device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
                                 uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
@@ -400,10 +409,57 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
     }
     const uint32_t nat = (n + 31) / 32;
     // Two operand registers, each reloaded two tiles ahead (no copies); the
-    // table ends with three spare tiles for these look-ahead loads.
+    // table ends with four spare tiles for the look-ahead loads.
     const v4i* __restrict__ bt = reinterpret_cast<const v4i*>(t.btab) + (F.g_begin / 32) * 64;
     const uint32_t n_tiles = (F.g_end - F.g_begin) / 32;
     v4i b0 = bt[lane], b1 = bt[64 + lane];
+    if (n == kAT * 32) {
+      // Full flush, every A row a window: the eight products of a pair of
+      // guide tiles are issued back to back and AND-reduced as they complete,
+      // with one ballot per pair; a pair with a non-negative element (a
+      // candidate, rare) is redone per A-tile pair.
+      uint32_t ti = 0;
+      for (; ti + 1 < n_tiles; ti += 2) {
+        // Issue order is fixed: two products in flight at most, each reduced
+        // while the next one runs (the empty asm keeps each partial AND where
+        // it is), and the operand reloads issued right after the last product
+        // that reads them.
+        v16i p0 = fp4_mma(A[0], b0);
+        v16i p1 = fp4_mma(A[1], b0);
+        uint32_t all = and16(p0);
+        asm volatile("" : "+v"(all));
+        p0 = fp4_mma(A[2], b0);
+        all &= and16(p1);
+        asm volatile("" : "+v"(all));
+        p1 = fp4_mma(A[3], b0);
+        b0 = bt[(ti + 2) * 64 + lane];
+        all &= and16(p0);
+        asm volatile("" : "+v"(all));
+        p0 = fp4_mma(A[0], b1);
+        all &= and16(p1);
+        asm volatile("" : "+v"(all));
+        p1 = fp4_mma(A[1], b1);
+        all &= and16(p0);
+        asm volatile("" : "+v"(all));
+        p0 = fp4_mma(A[2], b1);
+        all &= and16(p1);
+        asm volatile("" : "+v"(all));
+        p1 = fp4_mma(A[3], b1);
+        b1 = bt[(ti + 3) * 64 + lane];
+        all &= and16(p0);
+        all &= and16(p1);
+        if (ballot(static_cast<int>(all) >= 0)) {
+          // the rare pair is redone from reloaded operands (the registers
+          // already hold the next pair), which also keeps its products from
+          // being held live across the common path
+          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, bt[ti * 64 + lane], F.g_begin + 32 * ti);
+          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, bt[(ti + 1) * 64 + lane],
+                   F.g_begin + 32 * ti + 32);
+        }
+      }
+      if (ti < n_tiles) fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, b0, F.g_begin + 32 * ti);
+      return;
+    }
     for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
       fp4_tile(a, t, s, L, head, n, cs, lane, nat, A, b0, F.g_begin + 32 * ti);
       b0 = bt[(ti + 2) * 64 + lane];
@@ -592,7 +648,7 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
 }
 
 template <int NF, int ENGINE>
-__global__ __launch_bounds__(kMaxBlock) void batched_scan_kernel(
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void batched_scan_kernel(
     Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
     const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
     const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gnm,
@@ -991,7 +1047,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   P->n_guides = n_guides;
 
   // MFMA guide operands (the tables end with spare all-zero tiles for the
-  // kernel's look-ahead loads: one for int8, three for fp4).
+  // kernel's look-ahead loads: one for int8, four for fp4).
   //  * int8: per 32-guide tile, per k-half j and lane l (column l % 32, lane
   //    group l / 32) the 16 int8 of k = 32j + 16(l/32) .. +15 of the guide's
   //    tetrahedral code; k 60..63 hold the family's kmax - 15; padding
@@ -1023,7 +1079,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       }
     }
   } else if (P->tune.engine == 3) {
-    P->h_btab.assign((n_pad / 32 + 3) * 64 * 16, 0);
+    P->h_btab.assign((n_pad / 32 + 4) * 64 * 16, 0);
     for (const Fam& F : P->fams) {
       for (size_t gi = F.g_begin; gi < F.g_end; ++gi) {
         uint8_t nib[64] = {};
