@@ -232,10 +232,10 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // bases agree in all three elements or differ in exactly two, so the codes of
 // a window and a guide differ in 2 x (2-bit mismatches) elements.  Elements
 // 60..63 carry a bias that turns "within budget" into the sign of the
-// product.  One wave multiplies up to 64 buffered windows (two 32-row A
-// tiles, row = lane % 32) by 32-guide B tiles (column = lane % 32); a sign-bit
-// AND-reduction over the lane's 2 x 16 accumulators settles the common "no
-// site" case, and non-negative entries take the exact path.
+// product.  One wave multiplies up to ring_flush(ENGINE) buffered windows
+// (32-row A tiles, row = lane % 32) by 32-guide B tiles (column = lane % 32);
+// a sign-bit AND-reduction over the lane's accumulators settles the common
+// "no site" case, and non-negative entries take the exact path.
 //  * ENGINE 2, int8 (v_mfma_i32_32x32x32_i8, two steps of K = 32): elements
 //    are +1 (bit 0) / -1 (bit 1); guide elements 60..63 hold kmax - 15 and
 //    window ones +1, so a product is 4 * (kmax - mismatches).
@@ -245,7 +245,9 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 //    sum(a (2b - 1)) = |b| - 2 mm and |b| = 2 x (non-A guide bases), so the
 //    code part is (non-A guide bases) - mm; guide elements 60..63 sum to
 //    budget - (non-A guide bases) against window elements of 1.0, and the
-//    f32 product is exactly budget - mm.
+//    f32 product is exactly budget - mm.  Element 32 g + i of lane group g
+//    sits in dword i % 4 at nibble i / 4, so a window's operand dword d is
+//    (code word >> d) & 0x11111111.
 // Padding guides get a negative bias, so they never reach the exact path.
 // Lane layouts (checked on gfx950): A/B row or column = lane % 32; lane group
 // lane / 32 holds half of K and the hardware pairs equal (group, slot) of A
@@ -263,11 +265,10 @@ __device__ inline v4i expand16(uint32_t bits) {
   return r;
 }
 
-// 8 bits -> 8 fp4 nibbles of 0.5 * bit (bit i -> nibble i).
-__device__ inline uint32_t spread8(uint32_t b) {
-  b = (b | (b << 12)) & 0x000F000Fu;
-  b = (b | (b << 6)) & 0x03030303u;
-  return (b | (b << 3)) & 0x11111111u;
+// fp4 A operand dword d of a 32-bit code word: nibble k holds 0.5 * (bit
+// 4k + d of the word), OR-ed with the pad pattern.
+__device__ inline uint32_t fp4_dword(uint32_t c, uint32_t d, uint32_t pad) {
+  return ((c >> d) & 0x11111111u) | pad;
 }
 
 // Exact check of the (window, guide) pairs of one guide tile whose
@@ -399,13 +400,13 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
   }
   if constexpr (ENGINE == 3) {
     v8i A[4];
-    const uint32_t pad = g ? 0x22220000u : 0u;  // elements 60..63 = 1.0
+    const uint32_t pad = g ? 0x20000000u : 0u;  // elements 60..63 = 1.0
 #pragma unroll
     for (int tile = 0; tile < 4; ++tile) {
       const uint32_t c = g ? hi[tile] : lo[tile];
-      A[tile] = v8i{static_cast<int>(spread8(c & 0xFFu)), static_cast<int>(spread8((c >> 8) & 0xFFu)),
-                    static_cast<int>(spread8((c >> 16) & 0xFFu)),
-                    static_cast<int>(spread8(c >> 24) | pad), 0, 0, 0, 0};
+      A[tile] = v8i{static_cast<int>(fp4_dword(c, 0, pad)), static_cast<int>(fp4_dword(c, 1, pad)),
+                    static_cast<int>(fp4_dword(c, 2, pad)), static_cast<int>(fp4_dword(c, 3, pad)),
+                    0, 0, 0, 0};
     }
     const uint32_t nat = (n + 31) / 32;
     // Two operand registers, each reloaded two tiles ahead (no copies); the
@@ -1053,8 +1054,9 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   //    tetrahedral code; k 60..63 hold the family's kmax - 15; padding
   //    guides hold -127 there (never a candidate).
   //  * fp4: per 32-guide tile and lane l the 32 nibbles of elements
-  //    32(l/32) .. +31 (nibble i of dword i/8 at bits 4(i%8)); elements 60..63
-  //    sum to budget - (non-A bases); padding guides are 0 with -6 x 4 there.
+  //    32(l/32) .. +31 (element 32(l/32) + i in dword i%4 at bits 4(i/4), the
+  //    kernel's A layout); elements 60..63 sum to budget - (non-A bases);
+  //    padding guides are 0 with -6 x 4 there.
   P->h_btab.clear();
   if (P->tune.engine == 2) {
     P->h_btab.assign((n_pad / 32 + 1) * 2 * 64 * 16, 0);
@@ -1095,7 +1097,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
         const size_t tile = gi / 32, col = gi % 32;
         for (int grp = 0; grp < 2; ++grp) {
           uint32_t w[4] = {0, 0, 0, 0};
-          for (int i = 0; i < 32; ++i) w[i / 8] |= static_cast<uint32_t>(nib[32 * grp + i]) << (4 * (i % 8));
+          for (int i = 0; i < 32; ++i) w[i % 4] |= static_cast<uint32_t>(nib[32 * grp + i]) << (4 * (i / 4));
           std::memcpy(P->h_btab.data() + (tile * 64 + grp * 32 + col) * 16, w, 16);
         }
       }
