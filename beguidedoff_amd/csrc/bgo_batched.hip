@@ -411,9 +411,14 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
     const uint32_t nat = (n + 31) / 32;
     // Two operand registers, each reloaded two tiles ahead (no copies); the
     // table ends with four spare tiles for the look-ahead loads.
-    const v4i* __restrict__ bt = reinterpret_cast<const v4i*>(t.btab) + (F.g_begin / 32) * 64;
+    // tile base scalar, lane offset in a VGPR (saddr loads, no per-load VALU)
+    const char* __restrict__ bt = reinterpret_cast<const char*>(t.btab) + (F.g_begin / 32) * 1024;
+    const uint32_t l16 = lane * 16u;
+    auto btile = [&](uint32_t tile) {
+      return *reinterpret_cast<const v4i*>(bt + tile * 1024u + l16);
+    };
     const uint32_t n_tiles = (F.g_end - F.g_begin) / 32;
-    v4i b0 = bt[lane], b1 = bt[64 + lane];
+    v4i b0 = btile(0), b1 = btile(1);
     if (n == kAT * 32) {
       // Full flush, every A row a window: the eight products of a pair of
       // guide tiles are issued back to back and AND-reduced as they complete,
@@ -433,7 +438,7 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
         all &= and16(p1);
         asm volatile("" : "+v"(all));
         p1 = fp4_mma(A[3], b0);
-        b0 = bt[(ti + 2) * 64 + lane];
+        b0 = btile(ti + 2);
         all &= and16(p0);
         asm volatile("" : "+v"(all));
         p0 = fp4_mma(A[0], b1);
@@ -446,15 +451,15 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
         all &= and16(p1);
         asm volatile("" : "+v"(all));
         p1 = fp4_mma(A[3], b1);
-        b1 = bt[(ti + 3) * 64 + lane];
+        b1 = btile(ti + 3);
         all &= and16(p0);
         all &= and16(p1);
         if (ballot(static_cast<int>(all) >= 0)) {
           // the rare pair is redone from reloaded operands (the registers
           // already hold the next pair), which also keeps its products from
           // being held live across the common path
-          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, bt[ti * 64 + lane], F.g_begin + 32 * ti);
-          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, bt[(ti + 1) * 64 + lane],
+          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, btile(ti), F.g_begin + 32 * ti);
+          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, btile(ti + 1),
                    F.g_begin + 32 * ti + 32);
         }
       }
@@ -463,10 +468,10 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
     }
     for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
       fp4_tile(a, t, s, L, head, n, cs, lane, nat, A, b0, F.g_begin + 32 * ti);
-      b0 = bt[(ti + 2) * 64 + lane];
+      b0 = btile(ti + 2);
       if (ti + 1 < n_tiles)
         fp4_tile(a, t, s, L, head, n, cs, lane, nat, A, b1, F.g_begin + 32 * ti + 32);
-      b1 = bt[(ti + 3) * 64 + lane];
+      b1 = btile(ti + 3);
     }
   } else {
     const bool two = n > 32;
@@ -508,21 +513,25 @@ struct WaveState {
 template <int ENGINE>
 __device__ inline void append_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
                                      uint32_t s, uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
-                                     uint32_t cs, uint32_t lane) {
+                                     uint32_t cb, uint32_t lane) {
   constexpr uint32_t kSize = ring_size(ENGINE), kFlush = ring_flush(ENGINE);
   uint2* L = w.ring<ENGINE>(f);
   uint32_t tail = uniform(w.tails[f]);
   uint32_t head = uniform(w.heads[f]);
-  const uint32_t off0 = (bb << 5) - cs;
+  // Window offsets are from the 32-aligned base cb, so the block's offset
+  // is a multiple of 32 and the entry's offset fields are block constants
+  // plus the bit index.
+  const uint32_t off0 = (bb << 5) - cb;
+  const uint32_t x0 = off0 << 20, y0 = (off0 >> 12) << 20;
   for (;;) {
     const uint64_t act = ballot(pw != 0);
     if (!act) break;
     if (tail - head > kSize - 64) {  // no room for a full step: compare a batch
       wave_sync();
       if constexpr (ENGINE >= 2)
-        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
+        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
       else
-        compare_list(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
+        compare_list(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
       head += kFlush;
       if (lane == 0) w.wstats[f] += kFlush;
       wave_sync();
@@ -531,19 +540,18 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
     if (pw) {
       const uint32_t j = __builtin_ctz(pw);
       pw &= pw - 1;
-      const uint32_t off = off0 + j;
-      const uint32_t slot = (tail + lanes_below(act)) & (kSize - 1);
-      L[slot] = make_uint2((funnel(qb.x, qa.x, j) & kWin) | (off << 20),
-                           (funnel(qb.y, qa.y, j) & kWin) | ((off >> 12) << 20));
+      const uint32_t slot = lanes_below(act, tail) & (kSize - 1);
+      L[slot] = make_uint2((funnel(qb.x, qa.x, j) & kWin) | x0 | (j << 20),
+                           (funnel(qb.y, qa.y, j) & kWin) | y0);
     }
     tail += static_cast<uint32_t>(__popcll(act));
   }
   wave_sync();
   while (tail - head >= kFlush) {
     if constexpr (ENGINE >= 2)
-      compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
+      compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
     else
-      compare_list(a, t, t.fam[f], s, L, head, kFlush, cs, lane);
+      compare_list(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
     head += kFlush;
     if (lane == 0) w.wstats[f] += kFlush;
     wave_sync();
@@ -575,6 +583,7 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
   }
   wave_sync();
   const uint32_t b_first = cs >> 5, b_last = (ce - 1) >> 5;
+  const uint32_t cb = b_first << 5;  // ring entry offsets are from here
   // Each lane loads one plane word per step, of block min(b, b_last + 1) (the
   // plane array extends past every scanned block).  The following block's
   // word comes from lane + 1 (DPP wave_shl:1); lane 63 takes it from a
@@ -630,7 +639,7 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
 #pragma nounroll
     for (uint32_t f = 0; f < NF; ++f) {
       const uint32_t pw = w.pws[f * 64 + lane];
-      if (ballot(pw != 0)) append_family<ENGINE>(a, t, w, f, s, pw, bb, qa, qb, cs, lane);
+      if (ballot(pw != 0)) append_family<ENGINE>(a, t, w, f, s, pw, bb, qa, qb, cb, lane);
     }
   }
 #pragma nounroll
@@ -639,9 +648,9 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
     const uint32_t c = uniform(w.tails[f]) - head;
     if (c) {
       if constexpr (ENGINE >= 2)
-        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cs, lane);
+        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cb, lane);
       else
-        compare_list(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cs, lane);
+        compare_list(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cb, lane);
       if (lane == 0) w.wstats[f] += c;
     }
   }
@@ -901,7 +910,9 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   if (t) r = *t;
   if (r.chunk_positions <= 0) r.chunk_positions = 131072;
   r.chunk_positions = std::max(64, (r.chunk_positions + 63) / 64 * 64);
-  if (r.chunk_positions > (1 << 24)) return fail(BGO_ERR_ARG, "chunk_positions must be <= 2^24");
+  // ring entries hold 24-bit offsets from the 32-aligned chunk start
+  if (r.chunk_positions > (1 << 24) - 64)
+    return fail(BGO_ERR_ARG, "chunk_positions must be <= 2^24 - 64");
   if (r.waves_per_block <= 0) r.waves_per_block = 4;
   if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
     return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");

@@ -84,9 +84,9 @@ __device__ inline uint32_t lane_id() {
 }
 
 // Number of set bits of `mask` below this lane.
-__device__ inline uint32_t lanes_below(uint64_t mask) {
+__device__ inline uint32_t lanes_below(uint64_t mask, uint32_t plus = 0) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), plus));
 }
 
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
