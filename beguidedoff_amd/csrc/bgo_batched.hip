@@ -103,7 +103,7 @@ __host__ __device__ constexpr uint32_t ring_flush(int engine) { return engine ==
 // Bytes of LDS one wave uses with nf families (see the kernel's layout).
 __host__ __device__ constexpr size_t wave_lds_bytes(int nf, int engine) {
   return static_cast<size_t>(nf) * ring_size(engine) * 8 + kMaxFam * 8 +
-         static_cast<size_t>(nf) * 64 * 4 + 2 * kMaxFam * 4;
+         static_cast<size_t>(nf) * 64 * 4 + 2 * kMaxFam * 4 + 64 * 8;
 }
 
 __device__ inline uint32_t slot_pos(uint2 e) { return (e.x >> 20) | ((e.y >> 20) << 12); }
@@ -499,6 +499,7 @@ struct WaveState {
   uint32_t* heads;               // [kMaxFam] consumed ring index
   uint32_t* tails;               // [kMaxFam] appended ring index
   unsigned long long* wstats;    // [kMaxFam] windows compared
+  uint2* stage;                  // [64] plane words of the next step's blocks
   template <int ENGINE>
   __device__ uint2* ring(uint32_t f) const {
     return lists + f * ring_size(ENGINE);
@@ -585,20 +586,24 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
   const uint32_t b_first = cs >> 5, b_last = (ce - 1) >> 5;
   const uint32_t cb = b_first << 5;  // ring entry offsets are from here
   // Each lane loads one plane word per step, of block min(b, b_last + 1) (the
-  // plane array extends past every scanned block).  The following block's
-  // word comes from lane + 1 (DPP wave_shl:1); lane 63 takes it from a
-  // wave-uniform (scalar) load.  Both load two steps ahead.
+  // plane array extends past every scanned block), two steps ahead: the word
+  // loaded in step k is stored to the wave's LDS stage in step k + 1 and read
+  // back in step k + 2, so one register set carries it and no load is waited
+  // for in the step that issued it.  The following block's word comes from
+  // lane + 1 (DPP wave_shl:1); lane 63 reads it from the stage (lane 0's
+  // entry of the next step's blocks).  LDS operations of a wave complete in
+  // order, so each read returns the entry before the store that follows it.
   const uint32_t b_end = b_last + 1;
-  uint2 w1 = planes[min(b_first + lane, b_end)], w2 = planes[min(b_first + 64 + lane, b_end)];
-  uint2 e1 = planes[min(b_first + 64, b_end)], e2 = planes[min(b_first + 128, b_end)];
+  uint2 nxt = planes[min(b_first + lane, b_end)];
+  w.stage[lane] = nxt;
+  nxt = planes[min(b_first + 64 + lane, b_end)];
   for (uint32_t b0 = b_first; b0 <= b_last; b0 += 64) {
     const uint32_t bb = b0 + lane;
     const bool inb = bb <= b_last;
-    const uint2 qa = w1, e = e1;
-    w1 = w2;
-    w2 = planes[min(b0 + 128 + lane, b_end)];
-    e1 = e2;
-    e2 = planes[min(b0 + 192, b_end)];
+    const uint2 qa = w.stage[lane];
+    w.stage[lane] = nxt;
+    nxt = planes[min(b0 + 128 + lane, b_end)];
+    const uint2 e = w.stage[0];
     const uint2 qb = make_uint2(
         static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(e.x),
                                                           static_cast<int>(qa.x), 0x130, 0xF,
@@ -671,7 +676,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t n_waves = blockDim.x >> 6;
   // LDS per wave: NF window rings (ring_size x 8 B), kMaxFam stats (8 B),
-  // NF*64 pass words, kMaxFam heads and tails (4 B each).
+  // NF*64 pass words, kMaxFam heads and tails (4 B each), 64 staged plane
+  // words (8 B each).
   constexpr size_t kRings = NF * ring_size(ENGINE) * 8;
   char* my = reinterpret_cast<char*>(lds) + wave * wave_lds_bytes(NF, ENGINE);
   WaveState w;
@@ -680,6 +686,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
   w.pws = reinterpret_cast<uint32_t*>(my + kRings + kMaxFam * 8);
   w.heads = w.pws + NF * 64;
   w.tails = w.heads + kMaxFam;
+  w.stage = reinterpret_cast<uint2*>(w.tails + kMaxFam);
   if (lane < kMaxFam) w.wstats[lane] = 0;
   wave_sync();
   for (;;) {
