@@ -281,6 +281,31 @@ def test_budget_and_composition_extremes(gpu, engine):
     assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
 
 
+@pytest.mark.parametrize("engine", ENGINES)
+def test_odd_guide_tile_counts_with_planted_sites(gpu, engine):
+    """Families of 2, 3 and 5 guide tiles (40, 70 and 130 guides): full
+    128-window batches take the fp4 pair path, with the odd last tile alone.
+    Near-copies of guides from every tile are planted, so candidates fall in
+    both tiles of a pair and in the last tile."""
+    from brute import IUPAC, plant
+    rng = np.random.default_rng(23)
+    contigs, _, _ = build_case(23, length=(60_000, 90_000), n_guides=1)
+    guides = []
+    for pam, n in (("NG", 40), ("NGG", 70), ("NNNRRT", 130)):
+        guides += [("".join(rng.choice(list("ACGT"), size=20)), pam) for _ in range(n)]
+    for gi in rng.choice(len(guides), size=60, replace=False):
+        q, pam = guides[int(gi)]
+        pam_seq = "".join(rng.choice(list(IUPAC[c])) for c in pam)
+        plant(rng, contigs, q, pam_seq, int(rng.integers(0, 4)), "+" if rng.random() < 0.5 else "-")
+    budgets = [int(rng.integers(2, 6)) for _ in guides]
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
+    want = oracle_set(genome, guides, None, mm_per=budgets)
+    assert len({g for g, *_ in want}) >= 40
+    assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
+
+
 def test_fetch_view_equals_fetch(gpu):
     """The page-locked fetch returns the same records as the copying fetch,
     also after the buffer grows and when capped."""
