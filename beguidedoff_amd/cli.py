@@ -24,7 +24,8 @@ FAST_MODE_MESSAGE = ("fast mode (CPU FM-index) is not provided by beguidedoff_am
 def _encode_genome(args) -> int:
     """cli.py:9-17: FASTA -> .npz artefact."""
     genome_converter.convert(args.fasta, args.output, include_rc=args.include_rc,
-                             include_n_mask=args.include_n_mask, verbose=True)
+                             include_n_mask=args.include_n_mask, verbose=True,
+                             engine=args.engine, device=args.device)
     return 0
 
 
@@ -72,6 +73,9 @@ def build_parser() -> argparse.ArgumentParser:
                      help="Include the reverse complement (always on, as in the reference)")
     enc.add_argument("--include-n-mask", action="store_true", default=True,
                      help="Include the N mask (always on, as in the reference)")
+    enc.add_argument("--engine", choices=["numpy", "hip"], default="numpy",
+                     help="Encode on the host (numpy) or with the GPU encoder kernels (hip)")
+    enc.add_argument("--device", type=int, default=0, help="HIP device for --engine hip")
     enc.set_defaults(func=_encode_genome)
 
     idx = sub.add_parser("build-index", help="(fast mode; not provided)")

@@ -35,8 +35,18 @@ def read_fasta(path: str) -> Iterator[Tuple[str, str]]:
 
 
 def encode_contigs(contigs: List[Tuple[str, str]], include_rc: bool = True,
-                   include_n_mask: bool = True) -> Dict[str, np.ndarray]:
-    """Contig strings -> artefact arrays (upper-cased; non-ACGT -> A)."""
+                   include_n_mask: bool = True, engine: str = "numpy",
+                   device: int = 0) -> Dict[str, np.ndarray]:
+    """Contig strings -> artefact arrays (upper-cased; non-ACGT -> A).
+
+    ``engine="hip"`` runs the encoder kernels of libbgo_hip.so on ``device``
+    (``bgo_encode_genome``); ``engine="numpy"`` encodes on the host.  Both
+    give identical arrays; neither falls back to the other.
+    """
+    if engine == "hip":
+        return _encode_hip(contigs, include_rc, include_n_mask, device)
+    if engine != "numpy":
+        raise ValueError(f"engine must be 'numpy' or 'hip', got {engine!r}")
     names, offsets, lengths, fwd, rc, nmask = [], [], [], [], [], []
     off = 0
     for name, seq in contigs:
@@ -66,11 +76,60 @@ def encode_contigs(contigs: List[Tuple[str, str]], include_rc: bool = True,
     return out
 
 
+def encode_hip(seq: np.ndarray, offsets: np.ndarray, lengths: np.ndarray, device: int = 0,
+               include_rc: bool = True, include_n_mask: bool = True):
+    """Sequence bytes (uint8, contigs back to back at ``offsets``/``lengths``)
+    -> (forward words, reverse-complement words or None, N mask or None,
+    kernel ms), by ``bgo_encode_genome`` on ``device``."""
+    import ctypes as C
+
+    from . import _lib
+
+    _lib.init(device)
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    lengths = np.ascontiguousarray(lengths, dtype=np.int64)
+    n = int(seq.size)
+    fwd = np.zeros((n + 15) // 16, dtype=np.uint32)
+    rc = np.zeros_like(fwd) if include_rc else None
+    nmask = np.zeros((n + 7) // 8, dtype=np.uint8) if include_n_mask else None
+    ms = C.c_float(0.0)
+    _lib.check(_lib.load().bgo_encode_genome(
+        int(device), _lib.ptr(seq), n, _lib.ptr(offsets), _lib.ptr(lengths), int(offsets.size),
+        _lib.ptr(fwd), _lib.ptr(rc), _lib.ptr(nmask), C.byref(ms)))
+    return fwd, rc, nmask, float(ms.value)
+
+
+def _encode_hip(contigs: List[Tuple[str, str]], include_rc: bool, include_n_mask: bool,
+                device: int) -> Dict[str, np.ndarray]:
+    raw = [seq.encode("latin-1", "replace") for _, seq in contigs]
+    lengths = np.array([len(r) for r in raw], dtype=np.int64)
+    offsets = np.zeros(len(raw), dtype=np.int64)
+    if len(raw) > 1:
+        offsets[1:] = np.cumsum(lengths)[:-1]
+    seq = np.frombuffer(b"".join(raw), dtype=np.uint8)
+    fwd, rc, nmask, _ = encode_hip(seq, offsets, lengths, device, include_rc, include_n_mask)
+    out = {
+        "genome_forward": fwd,
+        "chrom_names": np.array([name for name, _ in contigs]),
+        "chrom_offsets": offsets,
+        "chrom_lengths": lengths,
+        "total_bp": np.int64(seq.size),
+    }
+    if include_rc:
+        out["genome_rc"] = rc
+    if include_n_mask:
+        out["n_mask"] = nmask
+    return out
+
+
 def convert(fasta_path: str, output_path: str, include_rc: bool = True,
             include_n_mask: bool = True, chunk_size: int = 100_000_000,
-            verbose: bool = True) -> None:
-    """FASTA -> .npz artefact (beguidedoff/genome_converter.py:15-108)."""
-    arrays = encode_contigs(list(read_fasta(fasta_path)), include_rc, include_n_mask)
+            verbose: bool = True, engine: str = "numpy", device: int = 0) -> None:
+    """FASTA -> .npz artefact (beguidedoff/genome_converter.py:15-108); see
+    :func:`encode_contigs` for ``engine``."""
+    arrays = encode_contigs(list(read_fasta(fasta_path)), include_rc, include_n_mask,
+                            engine=engine, device=device)
     np.savez_compressed(output_path, **arrays)
     if verbose:
         mb = Path(output_path).stat().st_size / (1024 * 1024)

@@ -93,6 +93,52 @@ def copy_probe(mbytes: int) -> None:
     log(f"copy probe: 3 x {n} bytes")
 
 
+def genome_text(g: dict) -> np.ndarray:
+    """ASCII bytes of a generated genome: A C G T from the forward words, N
+    where the artefact's N mask is set (N runs are coded 0 in the words)."""
+    L = int(g["total_bp"])
+    lut = np.zeros(256, dtype=np.uint32)
+    for b in range(256):
+        lut[b] = sum(ord("ACGT"[(b >> (2 * k)) & 3]) << (8 * k) for k in range(4))
+    words = np.ascontiguousarray(g["genome_forward"], dtype=np.uint32)
+    text = np.empty(words.size * 16, dtype=np.uint8)
+    step = 1 << 24  # words per chunk
+    for a in range(0, words.size, step):
+        chunk = words[a:a + step].view(np.uint8)
+        text[a * 16:(a + chunk.size // 4) * 16] = np.take(lut, chunk).view(np.uint8)
+    text = text[:L]
+    nm = g["n_mask"]
+    bstep = 1 << 27  # mask bytes per chunk
+    for a in range(0, nm.size, bstep):
+        bits = np.unpackbits(nm[a:a + bstep])[: max(0, L - 8 * a)].view(bool)
+        text[8 * a:8 * a + bits.size][bits] = ord("N")
+    return text
+
+
+def encoder_probe(workload: str) -> dict:
+    """The GPU genome encoder (bgo_encode_genome) on the text of a generated
+    genome; its outputs must equal the generator's own arrays bit for bit."""
+    from beguidedoff_amd import synth
+    from beguidedoff_amd.genome_converter import encode_hip
+
+    g = synth.workload(workload).genome
+    t0 = time.perf_counter()
+    text = genome_text(g)
+    log(f"encoder probe: {text.size} bytes of text in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    fwd, rc, nm, kernel_ms = encode_hip(text, g["chrom_offsets"], g["chrom_lengths"])
+    wall = time.perf_counter() - t0
+    exact = bool(np.array_equal(fwd, g["genome_forward"]) and np.array_equal(rc, g["genome_rc"])
+                 and np.array_equal(nm, g["n_mask"]))
+    n = int(text.size)
+    algo = n + 2 * 4 * ((n + 15) // 16) + (n + 7) // 8  # read text; write fwd, rc, mask
+    return {"metric": "genome_encoder", "workload": workload, "bases": n,
+            "kernel_ms": round(kernel_ms, 3), "algorithmic_bytes": algo,
+            "achieved_gbps": round(algo / kernel_ms / 1e6, 1), "peak_gbps": 8000.0,
+            "frac": round(algo / kernel_ms / 1e6 / 8000.0, 3),
+            "call_s_with_pcie": round(wall, 3), "outputs_equal_generator": exact}
+
+
 def cpu_baseline(workload, max_mm: int, target_s: float = 12.0) -> dict:
     from oracle.scan import scan_strand
     from beguidedoff_amd.encoding import encode_dna, encode_guide_64bit
@@ -266,6 +312,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration; "
                          "a process of its own: torch carries its own HIP runtime)")
+    ap.add_argument("--encoder-probe", action="store_true",
+                    help="only time the GPU genome encoder on the workload's genome text")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -276,6 +324,9 @@ def main() -> int:
 
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+        return 0
+    if args.encoder_probe:
+        print(json.dumps(encoder_probe(args.workload)), flush=True)
         return 0
 
     from beguidedoff_amd import _lib, synth

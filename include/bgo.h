@@ -237,6 +237,22 @@ int bgo_scan_batched(bgo_genome* g, const bgo_guide_desc* guides,
                      const bgo_tuning* tuning, bgo_hit* out,
                      int64_t out_capacity, int64_t* n_hits_out);
 
+/* ---------------------------------------------------------- genome encoder
+ * FASTA sequence bytes -> genome artefact arrays on `device`; replaces the
+ * per-base loops of genome_converter.convert (genome_converter.py:15-108).
+ * seq: the contigs' bytes back to back (any case); contig k occupies
+ * [contig_offsets[k], +contig_lengths[k]), and the contigs must tile
+ * [0, n_bytes).  Outputs (host arrays, each may be NULL):
+ *   words_fwd[ceil(n/16)]: upper-cased A C G T -> 0 1 2 3, other bytes -> 0,
+ *                          base i of a word at bits 2i (encoding.py:36-46);
+ *   words_rc[ceil(n/16)]:  per-contig reverse complement (3 - code);
+ *   n_mask[ceil(n/8)]:     np.packbits of (upper-cased byte == 'N').
+ * kernel_ms (may be NULL) receives the encode kernels' time. */
+int bgo_encode_genome(int32_t device, const uint8_t* seq, int64_t n_bytes,
+                      const int64_t* contig_offsets, const int64_t* contig_lengths,
+                      int32_t n_contigs, uint32_t* words_fwd, uint32_t* words_rc,
+                      uint8_t* n_mask, float* kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif
