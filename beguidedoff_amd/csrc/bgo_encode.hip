@@ -67,17 +67,30 @@ __global__ void fwd_kernel(const uint8_t* __restrict__ seq, uint64_t n, uint64_t
 __global__ void rc_kernel(const uint32_t* __restrict__ fwd, uint64_t n, uint64_t n_words,
                           const int64_t* __restrict__ offsets, const int64_t* __restrict__ lengths,
                           int32_t n_contigs, uint32_t* __restrict__ words) {
+  // One binary search per block: the last contig with offset <= the block's
+  // first position (empty contigs share the next offset); each thread walks
+  // forward from there.
+  __shared__ int32_t k_block;
   const uint64_t w = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (threadIdx.x == 0) {
+    const uint64_t qb = static_cast<uint64_t>(blockIdx.x) * blockDim.x * 16;
+    int32_t lo = 0, hi = n_contigs;
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (static_cast<uint64_t>(offsets[mid]) <= qb) lo = mid; else hi = mid;
+    }
+    k_block = lo;
+  }
+  __syncthreads();
   if (w >= n_words) return;
   const uint64_t q0 = w * 16;
-  // last contig with offset <= q0 (empty contigs share the next offset)
-  int32_t lo = 0, hi = n_contigs;
-  while (hi - lo > 1) {
-    const int32_t mid = (lo + hi) >> 1;
-    if (static_cast<uint64_t>(offsets[mid]) <= q0) lo = mid; else hi = mid;
-  }
-  int32_t k = lo;
+  int32_t k = k_block;
   uint64_t off = static_cast<uint64_t>(offsets[k]), end = off + static_cast<uint64_t>(lengths[k]);
+  while (q0 >= end && k + 1 < n_contigs) {
+    ++k;
+    off = static_cast<uint64_t>(offsets[k]);
+    end = off + static_cast<uint64_t>(lengths[k]);
+  }
   uint32_t word = 0;
   if (q0 + 16 <= end) {
     const uint64_t first = 2 * off + (end - off) - 1 - (q0 + 15);  // forward base of q0 + 15
