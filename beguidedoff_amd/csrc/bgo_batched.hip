@@ -524,10 +524,12 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
   // plus the bit index.
   const uint32_t off0 = (bb << 5) - cb;
   const uint32_t x0 = off0 << 20, y0 = (off0 >> 12) << 20;
-  for (;;) {
-    const uint64_t act = ballot(pw != 0);
-    if (!act) break;
-    if (tail - head > kSize - 64) {  // no room for a full step: compare a batch
+  // Each pass appends the next window of every lane that has one; passes run
+  // back to back while the ring has room for a full pass (64 slots), and a
+  // batch is compared when it has not.
+  uint64_t act = ballot(pw != 0);
+  while (act) {
+    if (tail - head > kSize - 64) {
       wave_sync();
       if constexpr (ENGINE >= 2)
         compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
@@ -538,14 +540,18 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
       wave_sync();
       continue;
     }
-    if (pw) {
-      const uint32_t j = __builtin_ctz(pw);
-      pw &= pw - 1;
+    do {
+      // The entry is formed in every lane and stored by those with a window;
+      // pw & (pw - 1) leaves an empty word empty.
+      const uint32_t j = __builtin_ctz(pw | 0x80000000u);
       const uint32_t slot = lanes_below(act, tail) & (kSize - 1);
-      L[slot] = make_uint2((funnel(qb.x, qa.x, j) & kWin) | x0 | (j << 20),
-                           (funnel(qb.y, qa.y, j) & kWin) | y0);
-    }
-    tail += static_cast<uint32_t>(__popcll(act));
+      const uint2 e = make_uint2((funnel(qb.x, qa.x, j) & kWin) | (x0 | (j << 20)),
+                                 (funnel(qb.y, qa.y, j) & kWin) | y0);
+      if (pw) L[slot] = e;
+      pw &= pw - 1;
+      tail += static_cast<uint32_t>(__popcll(act));
+      act = ballot(pw != 0);
+    } while (act && tail - head <= kSize - 64);
   }
   wave_sync();
   while (tail - head >= kFlush) {
