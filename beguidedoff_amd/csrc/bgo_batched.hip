@@ -91,19 +91,21 @@ struct Tab {
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-// Per-family window rings of a wave: ring_size(engine) slots, compared in
-// batches of ring_flush(engine) windows (64 = one per lane for the VALU
-// compare and the int8 engine's two A tiles; 128 = four fp4 A tiles, so each
-// guide operand load serves 128 windows).  A slot is {b0 window bits |
-// offset bits 0..11 << 20, b1 window bits | offset bits 12..23 << 20}, the
-// offset counted from the run start.
-__host__ __device__ constexpr uint32_t ring_size(int engine) { return engine == 3 ? 256 : 128; }
+// Per-family window buffers of a wave: windows are appended at the end and
+// compared in batches of ring_flush(engine) windows from the front (64 = one
+// per lane for the VALU compare and the int8 engine's two A tiles; 128 = four
+// fp4 A tiles, so each guide operand load serves 128 windows).  A buffer
+// holds one batch plus one append pass (64 windows); after a batch the at
+// most 64 windows behind it move to the front, so indices never wrap.  A slot
+// is {b0 window bits | offset bits 0..11 << 20, b1 window bits | offset bits
+// 12..23 << 20}, the offset counted from the run's 32-aligned start.
 __host__ __device__ constexpr uint32_t ring_flush(int engine) { return engine == 3 ? 128 : 64; }
+__host__ __device__ constexpr uint32_t ring_size(int engine) { return ring_flush(engine) + 64; }
 
 // Bytes of LDS one wave uses with nf families (see the kernel's layout).
 __host__ __device__ constexpr size_t wave_lds_bytes(int nf, int engine) {
   return static_cast<size_t>(nf) * ring_size(engine) * 8 + kMaxFam * 8 +
-         static_cast<size_t>(nf) * 64 * 4 + 2 * kMaxFam * 4 + 64 * 8;
+         static_cast<size_t>(nf) * 64 * 4 + kMaxFam * 4 + 64 * 8;
 }
 
 __device__ inline uint32_t slot_pos(uint2 e) { return (e.x >> 20) | ((e.y >> 20) << 12); }
@@ -174,10 +176,9 @@ __device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s
 // Compare the first n (<= 64) windows of a family ring with all guides of
 // the family.  One window per lane; guides are wave-uniform.
 __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, uint32_t s,
-                                    const uint2* L, uint32_t head, uint32_t n, uint32_t cs,
-                                    uint32_t lane) {
+                                    const uint2* L, uint32_t n, uint32_t cs, uint32_t lane) {
   if (lane >= n) return;
-  const uint2 e = L[(head + lane) & (ring_size(1) - 1)];
+  const uint2 e = L[lane];
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
   const uint32_t pos = cs + slot_pos(e);
   const uint32_t kmax = F.kmax;
@@ -276,7 +277,7 @@ __device__ inline uint32_t fp4_dword(uint32_t c, uint32_t d, uint32_t pad) {
 // product with A tile abase (abase + 1).
 template <int ENGINE>
 __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                  uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
+                                  uint32_t n, uint32_t cs, uint32_t lane,
                                   uint32_t bits, uint32_t gt, uint32_t abase) {
   const uint32_t m = lane & 31u, g = lane >> 5;
   while (ballot(bits != 0)) {
@@ -290,7 +291,7 @@ __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const
       const uint32_t widx = 32u * (abase + tile) + row;
       gi = gt + m;
       if (widx < n) {
-        const uint2 e = L[(head + widx) & (ring_size(ENGINE) - 1)];
+        const uint2 e = L[widx];
         W0 = e.x & kWin;
         W1 = e.y & kWin;
         pos = cs + slot_pos(e);
@@ -311,7 +312,7 @@ __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const
 // one ballot.
 template <int ENGINE>
 __device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                  uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
+                                  uint32_t n, uint32_t cs, uint32_t lane,
                                   const v16i& r0, const v16i& r1, uint32_t gt, uint32_t abase) {
   uint32_t all = static_cast<uint32_t>(r0[0]);
 #pragma unroll
@@ -325,12 +326,12 @@ __device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const
     bits |= static_cast<uint32_t>(r0[r] >= 0) << r;
     bits |= static_cast<uint32_t>(r1[r] >= 0) << (16 + r);
   }
-  mfma_exact<ENGINE>(a, t, s, L, head, n, cs, lane, bits, gt, abase);
+  mfma_exact<ENGINE>(a, t, s, L, n, cs, lane, bits, gt, abase);
 }
 
 // int8: one 32-guide tile against the (up to) two 32-window A tiles.
 __device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                               uint32_t head, uint32_t n, uint32_t cs, uint32_t lane, bool two,
+                               uint32_t n, uint32_t cs, uint32_t lane, bool two,
                                const v4i (&A)[2][2], int4 b0, int4 b1, uint32_t gt) {
   const v16i zero = {};
   const v4i B0 = {b0.x, b0.y, b0.z, b0.w}, B1 = {b1.x, b1.y, b1.z, b1.w};
@@ -341,7 +342,7 @@ __device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const ui
     acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, zero, 0, 0, 0);
     acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
   }
-  mfma_signs<2>(a, t, s, L, head, n, cs, lane, acc0, acc1, gt, 0);
+  mfma_signs<2>(a, t, s, L, n, cs, lane, acc0, acc1, gt, 0);
 }
 
 // fp4 product of one A tile with one guide tile (scales 2^0), as 32-bit
@@ -364,17 +365,17 @@ __device__ inline uint32_t and16(const v16i& p) {
 
 // fp4: one 32-guide tile against the nat (1..4) 32-window A tiles in use.
 __device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                uint32_t head, uint32_t n, uint32_t cs, uint32_t lane,
+                                uint32_t n, uint32_t cs, uint32_t lane,
                                 uint32_t nat, const v8i (&A)[4], v4i b, uint32_t gt) {
   v16i p0 = fp4_mma(A[0], b);
   v16i p1 = v16i{} - 1;
   if (nat > 1) p1 = fp4_mma(A[1], b);
-  mfma_signs<3>(a, t, s, L, head, n, cs, lane, p0, p1, gt, 0);
+  mfma_signs<3>(a, t, s, L, n, cs, lane, p0, p1, gt, 0);
   if (nat > 2) {
     p0 = fp4_mma(A[2], b);
     p1 = v16i{} - 1;
     if (nat > 3) p1 = fp4_mma(A[3], b);
-    mfma_signs<3>(a, t, s, L, head, n, cs, lane, p0, p1, gt, 2);
+    mfma_signs<3>(a, t, s, L, n, cs, lane, p0, p1, gt, 2);
   }
 }
 
@@ -384,7 +385,7 @@ __device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const u
 // bounds.
 template <int ENGINE>
 __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam& F, uint32_t s,
-                                         const uint2* L, uint32_t head, uint32_t n, uint32_t cs,
+                                         const uint2* L, uint32_t n, uint32_t cs,
                                          uint32_t lane) {
   constexpr uint32_t kAT = ring_flush(ENGINE) / 32;  // A tiles
   const uint32_t m = lane & 31u, g = lane >> 5;
@@ -393,7 +394,7 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
   for (uint32_t tile = 0; tile < kAT; ++tile) {
     const uint32_t widx = 32u * tile + m;
     uint2 e = make_uint2(0u, 0u);
-    if (widx < n) e = L[(head + widx) & (ring_size(ENGINE) - 1)];
+    if (widx < n) e = L[widx];
     const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
     lo[tile] = W1 | (W0 << 20);
     hi[tile] = (W0 >> 12) | ((W0 ^ W1) << 8);
@@ -458,19 +459,19 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
           // the rare pair is redone from reloaded operands (the registers
           // already hold the next pair), which also keeps its products from
           // being held live across the common path
-          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, btile(ti), F.g_begin + 32 * ti);
-          fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, btile(ti + 1),
+          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti), F.g_begin + 32 * ti);
+          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti + 1),
                    F.g_begin + 32 * ti + 32);
         }
       }
-      if (ti < n_tiles) fp4_tile(a, t, s, L, head, n, cs, lane, 4, A, b0, F.g_begin + 32 * ti);
+      if (ti < n_tiles) fp4_tile(a, t, s, L, n, cs, lane, 4, A, b0, F.g_begin + 32 * ti);
       return;
     }
     for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
-      fp4_tile(a, t, s, L, head, n, cs, lane, nat, A, b0, F.g_begin + 32 * ti);
+      fp4_tile(a, t, s, L, n, cs, lane, nat, A, b0, F.g_begin + 32 * ti);
       b0 = btile(ti + 2);
       if (ti + 1 < n_tiles)
-        fp4_tile(a, t, s, L, head, n, cs, lane, nat, A, b1, F.g_begin + 32 * ti + 32);
+        fp4_tile(a, t, s, L, n, cs, lane, nat, A, b1, F.g_begin + 32 * ti + 32);
       b1 = btile(ti + 3);
     }
   } else {
@@ -487,17 +488,16 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
       const int4 c0 = b0, c1 = b1;
       b0 = bp[128 + lane];
       b1 = bp[192 + lane];
-      i8_tile(a, t, s, L, head, n, cs, lane, two, A, c0, c1, gt);
+      i8_tile(a, t, s, L, n, cs, lane, two, A, c0, c1, gt);
     }
   }
 }
 
 // Per-wave LDS state of the scan loop.
 struct WaveState {
-  uint2* lists;                  // [NF][ring_size] window rings
+  uint2* lists;                  // [NF][ring_size] window buffers
   uint32_t* pws;                 // [NF][64] pass words of the current blocks
-  uint32_t* heads;               // [kMaxFam] consumed ring index
-  uint32_t* tails;               // [kMaxFam] appended ring index
+  uint32_t* counts;              // [kMaxFam] windows in each buffer
   unsigned long long* wstats;    // [kMaxFam] windows compared
   uint2* stage;                  // [64] plane words of the next step's blocks
   template <int ENGINE>
@@ -505,6 +505,23 @@ struct WaveState {
     return lists + f * ring_size(ENGINE);
   }
 };
+
+// Compare the batch at the front of family f's buffer (ring_flush windows)
+// and move the at most 64 windows behind it to the front.
+template <int ENGINE>
+__device__ inline void flush_front(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
+                                   uint32_t s, uint2* L, uint32_t cb, uint32_t lane) {
+  constexpr uint32_t kFlush = ring_flush(ENGINE);
+  wave_sync();
+  if constexpr (ENGINE >= 2)
+    compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, kFlush, cb, lane);
+  else
+    compare_list(a, t, t.fam[f], s, L, kFlush, cb, lane);
+  const uint2 e = L[kFlush + lane];  // LDS operations of a wave complete in order
+  L[lane] = e;
+  if (lane == 0) w.wstats[f] += kFlush;
+  wave_sync();
+}
 
 // Append this lane's passing windows (bits of pw in block bb) to family f's
 // ring, comparing every ring_flush(ENGINE) buffered windows.  All lanes call
@@ -517,56 +534,36 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
                                      uint32_t cb, uint32_t lane) {
   constexpr uint32_t kSize = ring_size(ENGINE), kFlush = ring_flush(ENGINE);
   uint2* L = w.ring<ENGINE>(f);
-  uint32_t tail = uniform(w.tails[f]);
-  uint32_t head = uniform(w.heads[f]);
+  uint32_t cnt = uniform(w.counts[f]);
   // Window offsets are from the 32-aligned base cb, so the block's offset
   // is a multiple of 32 and the entry's offset fields are block constants
   // plus the bit index.
   const uint32_t off0 = (bb << 5) - cb;
   const uint32_t x0 = off0 << 20, y0 = (off0 >> 12) << 20;
   // Each pass appends the next window of every lane that has one; passes run
-  // back to back while the ring has room for a full pass (64 slots), and a
+  // back to back while the buffer has room for a full pass (64 slots), and a
   // batch is compared when it has not.
   uint64_t act = ballot(pw != 0);
   while (act) {
-    if (tail - head > kSize - 64) {
-      wave_sync();
-      if constexpr (ENGINE >= 2)
-        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
-      else
-        compare_list(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
-      head += kFlush;
-      if (lane == 0) w.wstats[f] += kFlush;
-      wave_sync();
+    if (cnt > kSize - 64) {
+      flush_front<ENGINE>(a, t, w, f, s, L, cb, lane);
+      cnt -= kFlush;
       continue;
     }
     do {
       // The entry is formed in every lane and stored by those with a window;
       // pw & (pw - 1) leaves an empty word empty.
       const uint32_t j = __builtin_ctz(pw | 0x80000000u);
-      const uint32_t slot = lanes_below(act, tail) & (kSize - 1);
+      const uint32_t slot = lanes_below(act, cnt);
       const uint2 e = make_uint2((funnel(qb.x, qa.x, j) & kWin) | (x0 | (j << 20)),
                                  (funnel(qb.y, qa.y, j) & kWin) | y0);
       if (pw) L[slot] = e;
       pw &= pw - 1;
-      tail += static_cast<uint32_t>(__popcll(act));
+      cnt += static_cast<uint32_t>(__popcll(act));
       act = ballot(pw != 0);
-    } while (act && tail - head <= kSize - 64);
+    } while (act && cnt <= kSize - 64);
   }
-  wave_sync();
-  while (tail - head >= kFlush) {
-    if constexpr (ENGINE >= 2)
-      compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
-    else
-      compare_list(a, t, t.fam[f], s, L, head, kFlush, cb, lane);
-    head += kFlush;
-    if (lane == 0) w.wstats[f] += kFlush;
-    wave_sync();
-  }
-  if (lane == 0) {
-    w.heads[f] = head;
-    w.tails[f] = tail;
-  }
+  if (lane == 0) w.counts[f] = cnt;
   wave_sync();
 }
 
@@ -584,10 +581,7 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
   for (int f = 0; f < NF; ++f) {
     nibs[f] = t.fam[f].nibs;
   }
-  if (lane < kMaxFam) {
-    w.tails[lane] = 0;
-    w.heads[lane] = 0;
-  }
+  if (lane < kMaxFam) w.counts[lane] = 0;
   wave_sync();
   const uint32_t b_first = cs >> 5, b_last = (ce - 1) >> 5;
   const uint32_t cb = b_first << 5;  // ring entry offsets are from here
@@ -655,13 +649,16 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
   }
 #pragma nounroll
   for (uint32_t f = 0; f < NF; ++f) {
-    const uint32_t head = uniform(w.heads[f]);
-    const uint32_t c = uniform(w.tails[f]) - head;
+    uint32_t c = uniform(w.counts[f]);
+    if (c > ring_flush(ENGINE)) {
+      flush_front<ENGINE>(a, t, w, f, s, w.ring<ENGINE>(f), cb, lane);
+      c -= ring_flush(ENGINE);
+    }
     if (c) {
       if constexpr (ENGINE >= 2)
-        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cb, lane);
+        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane);
       else
-        compare_list(a, t, t.fam[f], s, w.ring<ENGINE>(f), head, c, cb, lane);
+        compare_list(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane);
       if (lane == 0) w.wstats[f] += c;
     }
   }
@@ -681,8 +678,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t n_waves = blockDim.x >> 6;
-  // LDS per wave: NF window rings (ring_size x 8 B), kMaxFam stats (8 B),
-  // NF*64 pass words, kMaxFam heads and tails (4 B each), 64 staged plane
+  // LDS per wave: NF window buffers (ring_size x 8 B), kMaxFam stats (8 B),
+  // NF*64 pass words, kMaxFam buffer counts (4 B each), 64 staged plane
   // words (8 B each).
   constexpr size_t kRings = NF * ring_size(ENGINE) * 8;
   char* my = reinterpret_cast<char*>(lds) + wave * wave_lds_bytes(NF, ENGINE);
@@ -690,9 +687,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
   w.lists = reinterpret_cast<uint2*>(my);
   w.wstats = reinterpret_cast<unsigned long long*>(my + kRings);
   w.pws = reinterpret_cast<uint32_t*>(my + kRings + kMaxFam * 8);
-  w.heads = w.pws + NF * 64;
-  w.tails = w.heads + kMaxFam;
-  w.stage = reinterpret_cast<uint2*>(w.tails + kMaxFam);
+  w.counts = w.pws + NF * 64;
+  w.stage = reinterpret_cast<uint2*>(w.counts + kMaxFam);
   if (lane < kMaxFam) w.wstats[lane] = 0;
   wave_sync();
   for (;;) {
