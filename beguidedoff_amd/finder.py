@@ -14,9 +14,11 @@ Two scan engines produce the same hits:
   persistent launch of the batched kernel, which also returns the mismatch
   mask and CCTOP score of each site.
 
-Per-hit decoding (coordinates, sequences, scores, editability, sorting) is
-vectorised with numpy and reproduces the reference's per-hit Python loop
-(finder.py:193-244, 405-472, 545-566) value for value.
+Per-hit decoding (coordinates, sequences, scores, editability, sorting) runs
+over the hits of all keys at once with numpy and reproduces the reference's
+per-hit Python loop (finder.py:193-244, 405-472, 545-566) value for value.
+With the batched engine the site and PAM bases come from the bit planes in
+the hit records; the reported rows are checked against the genome.
 """
 from __future__ import annotations
 
@@ -93,6 +95,39 @@ class RawHits:
                        np.concatenate([p.n_mm for p in parts]),
                        np.concatenate([p.mm_mask for p in parts]) if fused else None,
                        np.concatenate([p.score for p in parts]) if fused else None)
+
+
+@dataclass
+class _AllHits:
+    """Raw sites of all scan keys in flat arrays: key index, scanned strand,
+    position in the scanned strand array, N-corrected mismatch count, and
+    (batched engine) the fused-epilogue mismatch mask, score and the bit
+    planes x0 / x1 of the 26 scanned bases from the site's start."""
+    key: np.ndarray
+    strand: np.ndarray
+    pos: np.ndarray
+    n_mm: np.ndarray
+    mm_mask: Optional[np.ndarray] = None
+    score: Optional[np.ndarray] = None
+    x0: Optional[np.ndarray] = None
+    x1: Optional[np.ndarray] = None
+
+    @staticmethod
+    def from_dict(scan_guides: Sequence["_ScanGuide"],
+                  raw: Dict["_ScanGuide", RawHits]) -> "_AllHits":
+        parts = [raw.get(sg, RawHits.empty()) for sg in scan_guides]
+        sizes = np.array([len(p) for p in parts], dtype=np.int64)
+        key = np.repeat(np.arange(len(parts), dtype=np.int64), sizes)
+        full = [p for p in parts if len(p)]
+        if not full:
+            z = np.zeros(0, dtype=np.int64)
+            return _AllHits(z, z.astype(np.int8), z, z)
+        fused = all(p.mm_mask is not None for p in full)
+        return _AllHits(key, np.concatenate([p.strand for p in full]),
+                        np.concatenate([p.pos for p in full]),
+                        np.concatenate([p.n_mm for p in full]),
+                        np.concatenate([p.mm_mask for p in full]) if fused else None,
+                        np.concatenate([p.score for p in full]) if fused else None)
 
 
 def _parse_device(device) -> Tuple[str, int]:
@@ -387,27 +422,29 @@ class beguidedoff:
         cap = (self.max_results_per_guide if self.max_results_per_guide is not None
                else _MAX_RESULTS_PER_GUIDE)
         hits = res.hits
-        order = np.lexsort((hits["pos"], hits["strand"], hits["guide"]))
-        hits = hits[order]
-        bounds = np.searchsorted(hits["guide"], np.arange(len(scan_guides) + 1), side="left")
-        raw, overflow, counts = {}, {}, {}
-        for k, sg in enumerate(scan_guides):
-            h = hits[bounds[k]:bounds[k + 1]]
-            counts[sg] = int(h.size)
-            keep = np.ones(h.size, dtype=bool)
-            over = False
-            for s in (0, 1):
-                idx = np.flatnonzero(h["strand"] == s)
-                if idx.size > cap:
-                    keep[idx[cap:]] = False
-                    over = True
-            if over:
-                h = h[keep]
-            raw[sg] = RawHits(h["strand"], h["pos"], h["n_mm"], h["mm_mask"], h["score"])
-            overflow[sg] = over
-            if over:
-                _warn_truncated(log, sg, counts[sg], cap, int(h.size))
-        return raw, overflow, counts
+        key = hits["guide"].astype(np.int64)
+        n_keys = len(scan_guides)
+        counts_k = np.bincount(key, minlength=n_keys)
+        grp = key * 2 + hits["strand"].astype(np.int64)
+        sizes = np.bincount(grp, minlength=2 * n_keys)
+        over_k = (sizes.reshape(n_keys, 2) > cap).any(axis=1)
+        if over_k.any():
+            # per (key, strand) group in position order: keep the first `cap`
+            order = np.lexsort((hits["pos"], grp))
+            hits, key, grp = hits[order], key[order], grp[order]
+            rank = np.arange(grp.size, dtype=np.int64) - (np.cumsum(sizes) - sizes)[grp]
+            keep = rank < cap
+            kept_k = np.bincount(key[keep], minlength=n_keys)
+            hits, key = hits[keep], key[keep]
+            for k in np.flatnonzero(over_k):
+                _warn_truncated(log, scan_guides[k], int(counts_k[k]), cap, int(kept_k[k]))
+        flat = _AllHits(key, hits["strand"].astype(np.int8), hits["pos"].astype(np.int64),
+                        hits["n_mm"].astype(np.int64), hits["mm_mask"].astype(np.uint32),
+                        hits["score"].astype(np.float64), hits["x0"].astype(np.uint32),
+                        hits["x1"].astype(np.uint32))
+        overflow = {sg: bool(over_k[k]) for k, sg in enumerate(scan_guides)}
+        counts = {sg: int(counts_k[k]) for k, sg in enumerate(scan_guides)}
+        return flat, overflow, counts
 
     def close(self) -> None:
         if self._genome_handle is not None:
@@ -473,118 +510,185 @@ def _position_to_chrom(offsets: np.ndarray, pos: np.ndarray):
     return idx, pos - offsets[idx]
 
 
-@dataclass
-class _KeyHits:
-    n: int
-    strand: np.ndarray
-    report: np.ndarray      # 0-based global forward report position
-    seq_codes: np.ndarray   # (n, 20)
-    pam_str: np.ndarray     # object
-    n_mm: np.ndarray
-    mm_mask: np.ndarray
-    score: np.ndarray
+_W20 = (1 << 20) - 1
 
 
-def decode_key_hits(finder: "beguidedoff", sg: _ScanGuide, raw: RawHits) -> _KeyHits:
-    """Coordinates, on-target exclusion, sequences, mismatch mask and score of
-    one key's raw hits (beguidedoff/finder.py:193-244, 424-439)."""
-    offs, lens = finder.chrom_offsets, finder.chrom_lengths
-    plen = len(sg.pam)
-    window = 20 + plen
-    strand = raw.strand
-    pos = raw.pos
-    leftmost = pos.copy()
-    report = pos.copy()
-    minus = strand == 1
-    if minus.any():
-        k, local_rc = _position_to_chrom(offs, pos[minus])
-        lm = offs[k] + (lens[k] - local_rc - window)
-        leftmost[minus] = lm
-        report[minus] = lm + window - 1
-    keep = np.ones(pos.size, dtype=bool)
-    if sg.on_target_chrom_idx is not None:
-        k, local = _position_to_chrom(offs, leftmost)
-        s_code = 0 if sg.on_target_strand == "+" else (1 if sg.on_target_strand == "-" else -9)
-        keep &= ~((k == sg.on_target_chrom_idx) & (local == sg.on_target_local_pos)
-                  & (strand == s_code))
-    strand, pos, report = strand[keep], pos[keep], report[keep]
-    n_mm = raw.n_mm[keep]
-    seq = np.zeros((pos.size, 20), dtype=np.uint8)
-    pam = np.zeros((pos.size, plen), dtype=np.uint8)
-    for s, _label, words, _nm in finder._strand_arrays():
-        sel = strand == s
-        if sel.any():
-            seq[sel] = _codes_at(words, pos[sel], 20)
-            pam[sel] = _codes_at(words, pos[sel] + 20, plen)
-    g = encode_dna(sg.guide_seq)[:20]
-    g = np.pad(g, (0, 20 - g.size))
-    diff = (seq != g[None, :])
-    mm_mask = (diff.astype(np.uint32) << np.arange(20, dtype=np.uint32)[None, :]).sum(
+def _bits26(bits2d: np.ndarray) -> np.ndarray:
+    """(n, 26) 0/1 values -> uint32 with bit i = column i."""
+    return (bits2d.astype(np.uint32) << np.arange(26, dtype=np.uint32)[None, :]).sum(
         axis=1, dtype=np.uint64).astype(np.uint32)
-    if raw.score is not None:
-        score = raw.score[keep]
-        if not np.array_equal(raw.mm_mask[keep], mm_mask):
-            raise RuntimeError("fused epilogue mismatch mask disagrees with the genome")
-    else:
-        score = cctop_scores(mm_mask)
-    return _KeyHits(int(pos.size), strand, report, seq, _strings(pam), n_mm, mm_mask, score)
 
 
 def assemble_results(finder: "beguidedoff", guides: List[GuideInput],
-                     scan_guides: List[_ScanGuide], mapping, raw: Dict[_ScanGuide, RawHits],
+                     scan_guides: List[_ScanGuide], mapping, raw,
                      count_score_threshold: Optional[float], return_all: bool) -> pd.DataFrame:
-    """Per-input-guide sort, top-N and table (beguidedoff/finder.py:405-472, 545-566).
+    """Coordinates, on-target exclusion, sequences, scores, editability,
+    per-id sort, top-N and table for all keys at once
+    (beguidedoff/finder.py:193-244, 405-472, 545-566).
 
+    `raw` is an :class:`_AllHits` or a dict of per-key :class:`RawHits`.
     Hits are collected per guide_id (inputs sharing an id are merged, as the
     reference's id-keyed dicts merge them) and ordered by (score, chromosome
     name, position, strand); remaining ties keep the reference's insertion
-    order (scan key, then input order).
+    order (scan key, then input order).  Each input guide contributes its
+    id's rows, in input order.
     """
-    offs = finder.chrom_offsets
+    h = raw if isinstance(raw, _AllHits) else _AllHits.from_dict(scan_guides, raw)
+    offs, lens = finder.chrom_offsets, finder.chrom_lengths
     names = finder._chrom_names_arr
-    uniq = sorted(set(finder.chrom_names))
-    rank_of = {n: i for i, n in enumerate(uniq)}
+    rank_of = {n: i for i, n in enumerate(sorted(set(finder.chrom_names)))}
     chrom_rank = np.array([rank_of[n] for n in finder.chrom_names], dtype=np.int64)
-    key_index = {sg: i for i, sg in enumerate(scan_guides)}
-    decoded = {sg: decode_key_hits(finder, sg, raw.get(sg, RawHits.empty())) for sg in scan_guides}
+    plen_k = np.array([len(sg.pam) for sg in scan_guides], dtype=np.int64)
+
+    # coordinates: '-' hits map back to the forward leftmost and report
+    # positions (finder.py:193-215)
+    key, strand, pos = h.key, h.strand, h.pos
+    window = 20 + plen_k[key]
+    leftmost, report = pos.copy(), pos.copy()
+    minus = strand == 1
+    if minus.any():
+        k, local_rc = _position_to_chrom(offs, pos[minus])
+        lm = offs[k] + (lens[k] - local_rc - window[minus])
+        leftmost[minus] = lm
+        report[minus] = lm + window[minus] - 1
+    # on-target exclusion by (chromosome, local leftmost, strand) (finder.py:217-219)
+    has_ot = np.array([sg.on_target_chrom_idx is not None for sg in scan_guides], dtype=bool)
+    keep = np.ones(pos.size, dtype=bool)
+    if has_ot.size and has_ot.any():
+        ot_chrom = np.array([sg.on_target_chrom_idx if sg.on_target_chrom_idx is not None else -1
+                             for sg in scan_guides], dtype=np.int64)
+        ot_local = np.array([sg.on_target_local_pos if sg.on_target_local_pos is not None else -1
+                             for sg in scan_guides], dtype=np.int64)
+        ot_code = np.array([0 if sg.on_target_strand == "+" else
+                            (1 if sg.on_target_strand == "-" else -9) for sg in scan_guides],
+                           dtype=np.int64)
+        k, local = _position_to_chrom(offs, leftmost)
+        keep = ~(has_ot[key] & (k == ot_chrom[key]) & (local == ot_local[key])
+                 & (strand == ot_code[key]))
+    if not keep.all():
+        key, strand, pos, report = key[keep], strand[keep], pos[keep], report[keep]
+    n_mm = h.n_mm[keep] if not keep.all() else h.n_mm
+    n = int(pos.size)
+
+    # 20-nt site and genomic PAM from the scanned strand (finder.py:221-244),
+    # as bit planes of its 26 bases: from the batched kernel's records, or read
+    # from the strand arrays
+    if h.x0 is not None:
+        x0 = h.x0[keep] if not keep.all() else h.x0
+        x1 = h.x1[keep] if not keep.all() else h.x1
+    else:
+        codes = np.zeros((n, 26), dtype=np.uint8)
+        for s, _label, words, _nm in finder._strand_arrays():
+            sel = strand == s
+            if sel.any():
+                codes[sel] = _codes_at(words, pos[sel], 26)
+        x0, x1 = _bits26(codes & 1), _bits26(codes >> 1)
+    gp0 = np.zeros(len(scan_guides), dtype=np.uint32)
+    gp1 = np.zeros(len(scan_guides), dtype=np.uint32)
+    for i, sg in enumerate(scan_guides):
+        g = encode_dna(sg.guide_seq)[:20].astype(np.uint32)
+        sh = np.arange(g.size, dtype=np.uint32)
+        gp0[i] = int(((g & 1) << sh).sum())
+        gp1[i] = int(((g >> 1) << sh).sum())
+    mm_mask = ((x0 ^ gp0[key]) | (x1 ^ gp1[key])) & np.uint32(_W20)
+    if h.score is not None:
+        score = h.score[keep] if not keep.all() else h.score
+        fused = h.mm_mask[keep] if not keep.all() else h.mm_mask
+        if not np.array_equal(fused, mm_mask):
+            raise RuntimeError("fused epilogue mismatch mask disagrees with the site's bases")
+    else:
+        score = cctop_scores(mm_mask)
+    is_a = ~(x0 | x1) & np.uint32(_W20)   # code 0
+    is_c = x0 & ~x1 & np.uint32(_W20)     # code 1
+
+    # (hit, input guide) pairs: every input mapped to the hit's key (finder.py:424-439)
     input_index = {id(g): i for i, g in enumerate(guides)}
+    n_in = np.array([len(mapping[sg]) for sg in scan_guides], dtype=np.int64)
+    in_flat = np.array([input_index[id(g)] for sg in scan_guides for g in mapping[sg]],
+                       dtype=np.int64)
+    in_start = np.cumsum(n_in) - n_in
+    rep_n = n_in[key]
+    hid = np.repeat(np.arange(n, dtype=np.int64), rep_n)
+    j = np.arange(hid.size, dtype=np.int64) - np.repeat(np.cumsum(rep_n) - rep_n, rep_n)
+    inp = in_flat[in_start[key[hid]] + j]
+    # per-input editor target and window, and guide_id index (first appearance)
+    used = np.zeros(len(guides), dtype=bool)
+    used[inp] = True
+    target = np.zeros(len(guides), dtype=np.uint8)
+    wmask = np.zeros(len(guides), dtype=np.uint32)
+    for i in np.flatnonzero(used):
+        g = guides[i]
+        base = EDITABLE_BASE.get(g.editor_type)
+        if base is None:
+            raise ValueError(f"Unknown editor_type: {g.editor_type}")
+        target[i] = {"A": 0, "C": 1}[base]
+        wmask[i] = edit_window_mask(g.edit_window_start, g.edit_window_end)
+    id_of: Dict[str, int] = {}
+    gid = np.array([id_of.setdefault(g.guide_id, len(id_of)) for g in guides], dtype=np.int64)
+    n_ids = len(id_of)
+    editable = (np.where(target[inp] == 0, is_a[hid], is_c[hid]) & wmask[inp]) != 0
+    pid = gid[inp]
+    below = np.zeros(n_ids, dtype=np.int64)
+    if count_score_threshold is not None:
+        below = np.bincount(pid[editable & (score[hid] <= count_score_threshold)],
+                            minlength=n_ids).astype(np.int64)
+    take = np.flatnonzero(editable | bool(return_all))
+    hid, inp, pid = hid[take], inp[take], pid[take]
 
-    per_id: Dict[str, list] = defaultdict(list)
-    below: Dict[str, int] = defaultdict(int)
-    for sg in scan_guides:
-        kh = decoded[sg]
-        if kh.n == 0:
-            continue
-        for g in mapping[sg]:
-            target = EDITABLE_BASE.get(g.editor_type)
-            if target is None:
-                raise ValueError(f"Unknown editor_type: {g.editor_type}")
-            wmask = edit_window_mask(g.edit_window_start, g.edit_window_end)
-            base_is = kh.seq_codes == {"A": 0, "C": 1}[target]
-            bits = (base_is.astype(np.uint32) << np.arange(20, dtype=np.uint32)[None, :]).sum(
-                axis=1, dtype=np.uint64).astype(np.uint32)
-            editable = (bits & np.uint32(wmask)) != 0
-            if count_score_threshold is not None:
-                below[g.guide_id] += int(np.count_nonzero(
-                    editable & (kh.score <= count_score_threshold)))
-            idx = np.flatnonzero(editable | return_all)
-            if idx.size:
-                per_id[g.guide_id].append((key_index[sg], input_index[id(g)], sg, g, idx))
-
-    blocks = []
-    cache: Dict[str, Optional[Dict[str, np.ndarray]]] = {}
-    for g in guides:
-        gid = g.guide_id
-        if gid not in cache:
-            cache[gid] = _id_columns(finder, decoded, per_id.get(gid, []), below.get(gid, 0),
-                                     return_all, chrom_rank, names, offs)
-        if cache[gid] is not None:
-            blocks.append(cache[gid])
-    if not blocks:
+    # per-id order (score, chromosome name, position, strand, key, input)
+    chrom, local = _position_to_chrom(offs, report[hid])
+    # (chromosome rank, position) and (strand, key, input) packed into int64 keys
+    place = (chrom_rank[chrom] << 33) + local
+    tie = (strand[hid].astype(np.int64) << 62) + (key[hid] << 31) + inp
+    order = np.lexsort((tie, place, score[hid], pid))
+    pid_s = pid[order]
+    per_id = np.bincount(pid_s, minlength=n_ids)
+    starts = np.cumsum(per_id) - per_id
+    rank = np.arange(order.size, dtype=np.int64) - starts[pid_s]
+    if not return_all:
+        top = rank < finder.n_top_offtargets
+        order, pid_s, rank = order[top], pid_s[top], rank[top]
+        per_id = np.bincount(pid_s, minlength=n_ids)
+        starts = np.cumsum(per_id) - per_id
+    # rows: each input guide contributes its id's block, in input order
+    rows = np.concatenate([np.arange(starts[gid[i]], starts[gid[i]] + per_id[gid[i]])
+                           for i in range(len(guides))]) if len(guides) else np.zeros(0, np.int64)
+    if rows.size == 0:
         return pd.DataFrame()
-    # one table for all guides: the per-guide blocks in input order
-    return pd.DataFrame({c: np.concatenate([b[c] for b in blocks]) for c in OUTPUT_COLUMNS},
-                        columns=OUTPUT_COLUMNS)
+    r = order[rows]
+    rh, rk, rc = hid[r], key[hid[r]], chrom[r]
+    shifts = np.arange(26, dtype=np.uint32)
+    codes = (((x0[rh][:, None] >> shifts) & 1) | (((x1[rh][:, None] >> shifts) & 1) << 1)).astype(
+        np.uint8)
+    if h.x0 is not None:  # the reported sites' bases against the strand arrays
+        for s, _label, words, _nm in finder._strand_arrays():
+            sel = strand[rh] == s
+            if sel.any():
+                ref = _codes_at(words, pos[rh[sel]], 26)
+                width = 20 + plen_k[rk[sel]]
+                if np.any((ref != codes[sel]) & (np.arange(26)[None, :] < width[:, None])):
+                    raise RuntimeError("site bases in the hit records disagree with the genome")
+    gseq_k = np.array([sg.guide_seq for sg in scan_guides], dtype=object)
+    editor_in = np.array([g.base_editor or "" for g in guides], dtype=object)
+    pam_str = np.empty(rows.size, dtype=object)
+    for pl in np.unique(plen_k[rk]):
+        m = plen_k[rk] == pl
+        pam_str[m] = _strings(codes[m, 20:20 + pl])
+    cols = {
+        "guide_rna_sequence": gseq_k[rk],
+        "base_editor": editor_in[inp[r]],
+        "ot_rank": rank[rows] + 1,
+        "ot_chromosome": names[rc],
+        "ot_position": (local[r] + 1).astype(np.int64),
+        "ot_strand": np.where(strand[rh] == 0, "+", "-").astype(object),
+        "ot_sequence": _strings(codes[:, :20]),
+        "ot_pam": pam_str,
+        "n_mismatches": n_mm[rh].astype(np.int64),
+        "mismatch_positions": np.array([_mm_string(int(m)) for m in mm_mask[rh]], dtype=object),
+        "score": np.array([_round6(float(x)) for x in score[rh]], dtype=np.float64),
+        "n_offtargets_below_threshold": below[pid_s[rows]],
+    }
+    return pd.DataFrame(cols, columns=OUTPUT_COLUMNS)
 
 
 _ROUND_CACHE: Dict[float, float] = {}
@@ -597,48 +701,3 @@ def _round6(x: float) -> float:
         r = round(x, 6)
         _ROUND_CACHE[x] = r
     return r
-
-
-def _id_columns(finder, decoded, parts, n_below, return_all, chrom_rank, names, offs):
-    """Output columns of one guide id: its hits sorted by (score, chromosome
-    name, position, strand, scan key, input order), top-N unless return_all."""
-    if not parts:
-        return None
-    cols = defaultdict(list)
-    for kidx, iidx, sg, g, idx in parts:
-        kh = decoded[sg]
-        k, local = _position_to_chrom(offs, kh.report[idx])
-        cols["score"].append(kh.score[idx])
-        cols["crank"].append(chrom_rank[k])
-        cols["chrom"].append(k)
-        cols["local"].append(local)
-        cols["strand"].append(kh.strand[idx])
-        cols["key"].append(np.full(idx.size, kidx))
-        cols["inp"].append(np.full(idx.size, iidx))
-        cols["seq"].append(kh.seq_codes[idx])
-        cols["pam"].append(kh.pam_str[idx])
-        cols["n_mm"].append(kh.n_mm[idx])
-        cols["mm"].append(kh.mm_mask[idx])
-        cols["gseq"].append(np.full(idx.size, sg.guide_seq, dtype=object))
-        cols["editor"].append(np.full(idx.size, g.base_editor or "", dtype=object))
-    c = {k: np.concatenate(v) for k, v in cols.items()}
-    order = np.lexsort((c["inp"], c["key"], c["strand"], c["local"], c["crank"], c["score"]))
-    if not return_all:
-        order = order[: finder.n_top_offtargets]
-    if order.size == 0:
-        return None
-    return {
-        "guide_rna_sequence": c["gseq"][order],
-        "base_editor": c["editor"][order],
-        "ot_rank": np.arange(1, order.size + 1, dtype=np.int64),
-        "ot_chromosome": names[c["chrom"][order]],
-        "ot_position": (c["local"][order] + 1).astype(np.int64),
-        "ot_strand": np.where(c["strand"][order] == 0, "+", "-").astype(object),
-        "ot_sequence": _strings(c["seq"][order]),
-        "ot_pam": c["pam"][order],
-        "n_mismatches": c["n_mm"][order].astype(np.int64),
-        "mismatch_positions": np.array([_mm_string(int(m)) for m in c["mm"][order]],
-                                       dtype=object),
-        "score": np.array([_round6(float(x)) for x in c["score"][order]], dtype=np.float64),
-        "n_offtargets_below_threshold": np.full(order.size, n_below, dtype=np.int64),
-    }
