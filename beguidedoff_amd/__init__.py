@@ -9,15 +9,16 @@ library is loaded when a finder is created.
 """
 from .types import GuideInput, OffTarget
 
-__all__ = ["beguidedoff", "GuideInput", "OffTarget"]
+__all__ = ["beguidedoff", "beguidedofffast", "GuideInput", "OffTarget"]
 __version__ = "0.1.0"
 
 
 def beguidedoff(*args, mode: str = "extensive", **kwargs):
     """Create an off-target finder.
 
-    ``mode="extensive"`` (default) returns :class:`beguidedoff_amd.finder.beguidedoff`.
-    The reference's CPU FM-index ``mode="fast"`` is not part of this package.
+    ``mode="extensive"`` (default) returns :class:`beguidedoff_amd.finder.beguidedoff`;
+    ``mode="fast"`` returns :class:`beguidedoff_amd.fast.beguidedofffast`, the
+    reference's anchored search computed on the GPU without an FM-index.
     """
     mode = str(mode).lower()
     if mode == "extensive":
@@ -25,6 +26,14 @@ def beguidedoff(*args, mode: str = "extensive", **kwargs):
 
         return _extensive(*args, **kwargs)
     if mode == "fast":
-        raise ValueError("mode='fast' (CPU FM-index) is not provided by beguidedoff_amd; "
-                         "use mode='extensive'.")
+        from .fast import beguidedofffast as _fast
+
+        return _fast(*args, **kwargs)
     raise ValueError(f"Unknown mode {mode!r}; expected 'extensive' or 'fast'.")
+
+
+def beguidedofffast(*args, **kwargs):
+    """Shorthand for ``beguidedoff(mode="fast", ...)`` (beguidedoff/__init__.py:47-54)."""
+    from .fast import beguidedofffast as _fast
+
+    return _fast(*args, **kwargs)

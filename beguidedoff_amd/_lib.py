@@ -58,6 +58,7 @@ _SIGNATURES = {
     "bgo_plan_destroy": (C.c_int, [_P]),
     "bgo_plan_set_guides": (C.c_int, [_P, _P, C.c_int32]),
     "bgo_plan_set_hit_capacity": (C.c_int, [_P, C.c_int64]),
+    "bgo_plan_set_end_guard": (C.c_int, [_P, C.c_int32]),
     "bgo_plan_launch": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int32]),
     "bgo_plan_wait": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_float)]),
     "bgo_plan_fetch": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64)]),

@@ -160,6 +160,11 @@ class Plan:
         _lib.check(_lib.load().bgo_plan_set_guides(self.handle, _lib.ptr(self.descs),
                                                    self.n_guides))
 
+    def set_end_guard(self, end_guard: int) -> None:
+        """Scan positions < total_bp - end_guard (default 26, the reference's
+        extensive range)."""
+        _lib.check(_lib.load().bgo_plan_set_end_guard(self.handle, int(end_guard)))
+
     def set_hit_capacity(self, capacity: int) -> None:
         _lib.check(_lib.load().bgo_plan_set_hit_capacity(self.handle, int(capacity)))
         self.hit_capacity = int(capacity)
@@ -241,7 +246,8 @@ def default_capacity(n_guides: int) -> int:
 
 def scan(genome: DeviceGenome, specs: Sequence[GuideSpec], tuning: Optional[dict] = None,
          hit_capacity: Optional[int] = None, pos_begin: int = 0,
-         pos_end: Optional[int] = None, strands: int = 3) -> ScanResult:
+         pos_end: Optional[int] = None, strands: int = 3,
+         end_guard: Optional[int] = None) -> ScanResult:
     """Scan and return every site; re-runs once with an exact-size buffer when
     the first buffer was too small (the kernel counts past its capacity)."""
     cap = int(hit_capacity or default_capacity(len(specs)))
@@ -249,6 +255,8 @@ def scan(genome: DeviceGenome, specs: Sequence[GuideSpec], tuning: Optional[dict
     while True:
         plan = Plan(genome, specs, tuning=tuning, hit_capacity=cap)
         try:
+            if end_guard is not None:
+                plan.set_end_guard(end_guard)
             plan.launch(pos_begin, pos_end, strands)
             n, ms = plan.wait()
             launches += 1

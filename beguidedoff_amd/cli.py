@@ -5,8 +5,9 @@ subcommands and options (beguidedoff/cli.py:75-126):
     python -m beguidedoff_amd.cli search --genome G.npz --guides guides.tsv [--output out.tsv]
     python -m beguidedoff_amd.cli verify --encoded G.npz --fasta G.fa
 
-``search --mode fast`` and ``build-index`` (the CPU FM-index mode) are not part
-of this package; they exit with a message saying so.
+``search --mode fast`` runs the fast (anchored, score-cutoff) search on the GPU
+over an encoded genome, which is its index: ``build-index`` writes that
+artefact (the same file as ``encode-genome``), and ``--index`` names it.
 """
 from __future__ import annotations
 
@@ -17,10 +18,6 @@ from typing import List, Optional
 
 from . import genome_converter
 
-FAST_MODE_MESSAGE = ("fast mode (CPU FM-index) is not provided by beguidedoff_amd; "
-                     "use --mode extensive with an encoded genome")
-
-
 def _encode_genome(args) -> int:
     """cli.py:9-17: FASTA -> .npz artefact."""
     genome_converter.convert(args.fasta, args.output, include_rc=args.include_rc,
@@ -30,21 +27,29 @@ def _encode_genome(args) -> int:
 
 
 def _build_index(args) -> int:
-    raise SystemExit(FAST_MODE_MESSAGE)
+    """cli.py:20-24: the fast mode's index is the encoded genome artefact."""
+    genome_converter.convert(args.fasta, args.output, verbose=True)
+    return 0
 
 
 def _search(args) -> int:
     """cli.py:27-59: extensive search of a guide TSV; the table goes to
     --output as TSV, or to stdout."""
     if args.mode == "fast":
-        raise SystemExit(FAST_MODE_MESSAGE)
-    if not args.genome:
-        raise SystemExit("--genome is required for --mode extensive")
-    from .finder import beguidedoff
+        if not args.index:
+            raise SystemExit("--index is required for --mode fast")
+        from .fast import beguidedofffast
 
-    finder = beguidedoff(genome_path=args.genome, max_mismatches=args.max_mismatches,
-                         device=args.device, n_top_offtargets=args.n_top,
-                         results_per_guide=args.results_per_guide)
+        finder = beguidedofffast(index_path=args.index, search_cutoff=args.search_cutoff,
+                                 n_top_offtargets=args.n_top, device=args.device)
+    else:
+        if not args.genome:
+            raise SystemExit("--genome is required for --mode extensive")
+        from .finder import beguidedoff
+
+        finder = beguidedoff(genome_path=args.genome, max_mismatches=args.max_mismatches,
+                             device=args.device, n_top_offtargets=args.n_top,
+                             results_per_guide=args.results_per_guide)
     table = finder.search(input_file=args.guides, return_all=args.return_all,
                           count_score_threshold=args.count_score_threshold,
                           output_file=args.output, include_non_editable=args.include_non_editable,
@@ -78,18 +83,20 @@ def build_parser() -> argparse.ArgumentParser:
     enc.add_argument("--device", type=int, default=0, help="HIP device for --engine hip")
     enc.set_defaults(func=_encode_genome)
 
-    idx = sub.add_parser("build-index", help="(fast mode; not provided)")
-    idx.add_argument("--fasta", required=True)
-    idx.add_argument("--output", required=True)
+    idx = sub.add_parser("build-index",
+                         help="Build the fast mode's index (the encoded genome .npz)")
+    idx.add_argument("--fasta", required=True, help="Input FASTA file")
+    idx.add_argument("--output", required=True, help="Output .npz path")
     idx.set_defaults(func=_build_index)
 
     s = sub.add_parser("search", help="Search for off-targets")
     s.add_argument("--mode", choices=["extensive", "fast"], default="extensive")
     s.add_argument("--genome", default=None, help="Encoded genome .npz")
-    s.add_argument("--index", default=None, help="(fast mode; not provided)")
+    s.add_argument("--index", default=None, help="Encoded genome .npz (fast mode)")
     s.add_argument("--guides", required=True, help="Guide TSV input")
     s.add_argument("--max-mismatches", type=int, default=4)
-    s.add_argument("--search-cutoff", type=float, default=5.0, help="(fast mode; ignored)")
+    s.add_argument("--search-cutoff", type=float, default=5.0,
+                   help="CCTOP score cutoff T for fast mode (default: 5.0)")
     s.add_argument("--device", default=None, help="HIP device, e.g. cuda:0")
     s.add_argument("--n-top", type=int, default=3, help="Top N off-targets per guide")
     s.add_argument("--results-per-guide", type=int, default=10240,

@@ -759,6 +759,7 @@ using bgo::bk::Fam;
 struct bgo_plan {
   bgo_genome* g = nullptr;
   int device = 0;
+  int32_t end_guard = 26;             // positions >= total_bp - end_guard are not scanned
   int32_t n_guides = 0;
   int64_t hit_cap = 0;
   double score[21] = {};
@@ -1217,6 +1218,22 @@ extern "C" int bgo_plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, in
   return plan_set_guides(P, guides, n_guides);
 }
 
+extern "C" int bgo_plan_set_end_guard(bgo_plan* P, int32_t end_guard) {
+  if (!P || end_guard < 21 || end_guard > 64)
+    return fail(BGO_ERR_ARG, "end_guard must be in [21, 64]");
+  BGO_TRY(injected());
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  P->end_guard = end_guard;
+  if (P->graph) {
+    (void)hipGraphExecDestroy(P->graph);
+    P->graph = nullptr;
+    P->graph_mask = -1;
+  }
+  return BGO_OK;
+}
+
 extern "C" int bgo_plan_set_hit_capacity(bgo_plan* P, int64_t hit_capacity) {
   if (!P || hit_capacity < 0) return fail(BGO_ERR_ARG, "bad arguments");
   BGO_TRY(injected());
@@ -1244,7 +1261,7 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
   hipLaunchKernelGGL(bk::reset_counters_kernel, dim3(1), dim3(256), 0, P->stream, P->d_counters,
                      P->n_counter_words);
   BGO_HIP(hipGetLastError());
-  const int64_t hard_end = G->total_bp - 26;  // cuda_search.py:211-213
+  const int64_t hard_end = G->total_bp - P->end_guard;  // 26: cuda_search.py:211-213
   const int64_t vb = std::max<int64_t>(pos_begin, 0);
   const int64_t ve = std::min<int64_t>(pos_end, hard_end);
   uint32_t slots[2];

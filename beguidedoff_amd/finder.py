@@ -102,7 +102,9 @@ class _AllHits:
     """Raw sites of all scan keys in flat arrays: key index, scanned strand,
     position in the scanned strand array, N-corrected mismatch count, and
     (batched engine) the fused-epilogue mismatch mask, score and the bit
-    planes x0 / x1 of the 26 scanned bases from the site's start."""
+    planes x0 / x1 of the 26 scanned bases from the site's start.  mm_out,
+    when given, is the mask reported as mismatch positions (the fast mode's
+    includes the N positions); by default the 2-bit mask is reported."""
     key: np.ndarray
     strand: np.ndarray
     pos: np.ndarray
@@ -111,6 +113,7 @@ class _AllHits:
     score: Optional[np.ndarray] = None
     x0: Optional[np.ndarray] = None
     x1: Optional[np.ndarray] = None
+    mm_out: Optional[np.ndarray] = None
 
     @staticmethod
     def from_dict(scan_guides: Sequence["_ScanGuide"],
@@ -463,11 +466,14 @@ def _warn_truncated(log, sg, count, rpg, kept):
 
 
 def _codes_at(words: np.ndarray, pos: np.ndarray, width: int) -> np.ndarray:
-    """2-bit codes of bases [pos, pos + width) for every pos: (n, width) uint8."""
+    """2-bit codes of bases [pos, pos + width) for every pos: (n, width) uint8.
+    Bases past the end of the array read as its last word's (callers ignore
+    them)."""
     if pos.size == 0:
         return np.zeros((0, width), dtype=np.uint8)
+    words = np.asarray(words)
     idx = pos[:, None] + np.arange(width, dtype=np.int64)[None, :]
-    w = np.asarray(words)[idx >> 4].astype(np.uint32)
+    w = words[np.minimum(idx >> 4, words.size - 1)].astype(np.uint32)
     return ((w >> ((idx & 15) * 2).astype(np.uint32)) & 3).astype(np.uint8)
 
 
@@ -598,6 +604,10 @@ def assemble_results(finder: "beguidedoff", guides: List[GuideInput],
             raise RuntimeError("fused epilogue mismatch mask disagrees with the site's bases")
     else:
         score = cctop_scores(mm_mask)
+    if h.mm_out is not None:
+        mm_report = h.mm_out[keep] if not keep.all() else h.mm_out
+    else:
+        mm_report = mm_mask
     is_a = ~(x0 | x1) & np.uint32(_W20)   # code 0
     is_c = x0 & ~x1 & np.uint32(_W20)     # code 1
 
@@ -684,7 +694,8 @@ def assemble_results(finder: "beguidedoff", guides: List[GuideInput],
         "ot_sequence": _strings(codes[:, :20]),
         "ot_pam": pam_str,
         "n_mismatches": n_mm[rh].astype(np.int64),
-        "mismatch_positions": np.array([_mm_string(int(m)) for m in mm_mask[rh]], dtype=object),
+        "mismatch_positions": np.array([_mm_string(int(m)) for m in mm_report[rh]],
+                                       dtype=object),
         "score": np.array([_round6(float(x)) for x in score[rh]], dtype=np.float64),
         "n_offtargets_below_threshold": below[pid_s[rows]],
     }

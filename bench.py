@@ -237,7 +237,35 @@ def fast_mode_leg(workload, hits: np.ndarray, threads: int, cutoff: float = 5.0)
             win = 20 + len(x.pam)
             if n_ext.get(gi, False) or any(has_n(int(l), win) for l in r.leftmost.tolist()):
                 disagree_n += 1
+    # the same search on the GPU (beguidedoff_amd.fast): candidate sets per guide
+    from beguidedoff_amd.fast import beguidedofffast
+    from beguidedoff_amd.types import GuideInput
+
+    ff = beguidedofffast(genome=g, search_cutoff=cutoff)
+    inputs = [GuideInput(f"g{i}", s, p, "3prime", 1, 20, "ABE") for i, (s, p) in enumerate(guides)]
+    keys, _ = ff._ext._deduplicate_guides(inputs)
+    ff._candidates(keys, cutoff)  # genome upload and first launch
+    t0 = time.perf_counter()
+    fh = ff._candidates(keys, cutoff)
+    gpu_s = time.perf_counter() - t0
+    gpu_sets = {}
+    for kk, st, pos, mm, sc in zip(fh.key.tolist(), fh.strand.tolist(), fh.pos.tolist(),
+                                   fh.mm_out.tolist(), fh.score.tolist()):
+        win = 20 + len(keys[kk].pam)
+        if st == 1:
+            c = int(np.searchsorted(offs, pos, side="right") - 1)
+            pos = int(offs[c] + lens[c] - (pos - offs[c]) - win)
+        gpu_sets.setdefault((keys[kk].guide_seq, keys[kk].pam), set()).add((pos, st, mm, sc))
+    same = sum(gpu_sets.get(key, set()) == set(zip(r.leftmost.tolist(), r.strand.tolist(),
+                                                   r.mm.tolist(), r.score.tolist()))
+               for key, r in zip(guides, res))
+    ff._ext.close()
+
     gbp = g["total_bp"] / 1e9
+    gpu_fast = {"value": round(len(guides) * gbp / gpu_s, 1), "unit": "guide·Gbp/s",
+                "query_s": round(gpu_s, 4), "candidates": int(fh.key.size),
+                "guides_with_equal_candidate_sets": int(same), "guides": len(guides)}
+    log(f"GPU fast mode: {gpu_fast}")
     return {"value": round(len(guides) * gbp / qn, 1), "value_1t": round(len(guides) * gbp / q1, 1),
             "unit": "guide·Gbp/s", "cores": threads, "kind": "port",
             "sample": f"all {len(guides)} 20-nt guides, whole genome, cutoff {cutoff}",
@@ -245,7 +273,8 @@ def fast_mode_leg(workload, hits: np.ndarray, threads: int, cutoff: float = 5.0)
             "index_bytes": index_bytes,
             "verdicts": {"guides": len(guides), "agree": agree, "disagree": disagree,
                          "disagree_at_n_windows": disagree_n, "remove_fast": n_remove_fast,
-                         "remove_extensive": len(remove_ext), "anchor_cap_overflow": overflow}}
+                         "remove_extensive": len(remove_ext), "anchor_cap_overflow": overflow},
+            "gpu": gpu_fast}
 
 
 def serving_leg(dg, pool, max_mm: int, rates, duration: float, seed: int = 0) -> dict:

@@ -190,6 +190,14 @@ int bgo_plan_set_guides(bgo_plan* p, const bgo_guide_desc* guides,
  * previous launch; the old records are discarded). */
 int bgo_plan_set_hit_capacity(bgo_plan* p, int64_t hit_capacity);
 
+/* Positions >= total_bp - end_guard are not scanned.  The default, 26,
+ * reproduces the reference's scan range (cuda_search.py:211-213: the
+ * extensive mode skips the genome's last 26 positions).  The fast mode sets
+ * 20 + its shortest PAM, so every window that fits in the genome is scanned;
+ * windows that run past the genome end then read zero padding and are
+ * dropped by its contig-internal filter.  Range [21, 64]. */
+int bgo_plan_set_end_guard(bgo_plan* p, int32_t end_guard);
+
 /* Enqueue one scan of positions [pos_begin, pos_end) (clamped to
  * [0, total_bp - 26), cuda_search.py:211-213) on the strands of strand_mask
  * (bit0 '+', bit1 '-') on the plan's stream.  Returns without waiting. */

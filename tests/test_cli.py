@@ -38,13 +38,22 @@ def test_encode_genome_and_verify(tmp_path):
     assert cli.main(["verify", "--encoded", str(other), "--fasta", str(fa), "--sample-n", "500"]) == 1
 
 
-def test_fast_mode_and_missing_genome_exit_with_a_message(tmp_path):
-    with pytest.raises(SystemExit, match="fast mode"):
+def test_missing_genome_or_index_exits_with_a_message(tmp_path):
+    with pytest.raises(SystemExit, match="--index is required"):
         cli.main(["search", "--mode", "fast", "--guides", "g.tsv"])
-    with pytest.raises(SystemExit, match="fast mode"):
-        cli.main(["build-index", "--fasta", "g.fa", "--output", "g.fmi"])
     with pytest.raises(SystemExit, match="--genome is required"):
         cli.main(["search", "--guides", "g.tsv"])
+
+
+def test_build_index_writes_the_encoded_genome(tmp_path):
+    contigs = [("chr1", "ACGTNNACGTTTGCA" * 7), ("chr2", "GGGCCCAAATTT" * 5)]
+    fa = tmp_path / "g.fa"
+    _fasta(fa, contigs)
+    assert cli.main(["build-index", "--fasta", str(fa), "--output", str(tmp_path / "i.npz")]) == 0
+    got = load_genome(str(tmp_path / "i.npz"))
+    want = encode_contigs(contigs)
+    for key in ("genome_forward", "genome_rc", "n_mask"):
+        assert np.array_equal(got[key], want[key])
 
 
 @pytest.mark.gpu
@@ -64,3 +73,10 @@ def test_search_writes_the_finders_table(gpu, tmp_path):
     assert out_cli.read_text() == out_api.read_text()
     back = pd.read_csv(out_cli, sep="\t")
     assert list(back.columns) == list(df.columns) and len(back) == len(df)
+    # fast mode over the same artefact as its index
+    out_fast, out_fapi = tmp_path / "fast.tsv", tmp_path / "fapi.tsv"
+    assert cli.main(["search", "--mode", "fast", "--index", str(npz), "--guides", str(tsv),
+                     "--search-cutoff", "5", "--output", str(out_fast)]) == 0
+    make_finder(mode="fast", index_path=str(npz)).search(input_file=str(tsv),
+                                                          output_file=str(out_fapi), verbose=False)
+    assert out_fast.read_text() == out_fapi.read_text()

@@ -1,0 +1,209 @@
+"""The GPU fast mode (beguidedoff_amd/fast.py) against the fast-mode oracles:
+the table of oracle/fast_finder_oracle.py (20-nt guides, over the C
+restatement of the anchored candidate search) and a literal brute force of
+fast/finder.py:163-277 (20- and 21-nt guides).  Genomes carry N runs, sites
+planted at both genome tails, across contig junctions and with N inside."""
+import numpy as np
+import pandas as pd
+import pytest
+
+from beguidedoff_amd import beguidedoff as make_finder
+from beguidedoff_amd import synth
+from beguidedoff_amd.fast import cold_region, mismatch_budget
+from beguidedoff_amd.types import GuideInput
+from brute import IUPAC, plant, random_contigs, revcomp
+from oracle.fast_finder_oracle import FastFinderOracle
+
+IUPAC_BITS = {"A": 1, "C": 2, "G": 4, "T": 8, "R": 5, "Y": 10, "S": 6, "W": 9, "K": 12, "M": 3,
+              "B": 14, "D": 13, "H": 11, "V": 7, "N": 15}
+CODE = {"A": 0, "C": 1, "G": 2, "T": 3}
+
+
+def test_regions_and_budget_known_answers():
+    # reference tests/test_regions.py:7-14: cutoff 5 -> cold positions 9-20
+    assert cold_region(5.0) == (8, 12) and cold_region(3.0) == (6, 14)
+    # 1.2 + 1.44 + 1.728 = 4.368 <= 5 < 4.368 + 2.0736
+    assert [mismatch_budget(t) for t in (5.0, 4.368, 4.36, 3.0, 1.2, 1.19)] == [3, 3, 2, 2, 1, 0]
+    with pytest.raises(ValueError, match="cold"):
+        cold_region(40.0)
+
+
+def brute_candidates(contigs, guide, pam, cutoff):
+    """fast/finder.py:163-277 literally: anchors on the A-coded forward
+    genome, window and anchor inside one contig, IUPAC PAM, 2-bit mismatches
+    of the first 20 bases plus the N positions, score <= cutoff."""
+    full = "".join(s for _, s in contigs).upper()
+    enc = "".join(c if c in "ACGT" else "A" for c in full)
+    isn = [c == "N" for c in full]
+    offs = np.cumsum([0] + [len(s) for _, s in contigs])[:-1]
+    lens = [len(s) for _, s in contigs]
+    c0, m = cold_region(cutoff)
+    plen, window = len(pam), 20 + len(pam)
+    out = set()
+    for strand, anchor, boff in (("+", guide[c0:20], c0), ("-", revcomp(guide)[:m], 0)):
+        start = enc.find(anchor)
+        while start >= 0:
+            gpos = start
+            start = enc.find(anchor, start + 1)
+            lm = gpos - boff if strand == "+" else gpos - plen
+            k = int(np.searchsorted(offs, lm, side="right") - 1)
+            off, ln = int(offs[k]), lens[k]
+            if gpos < off or gpos + len(anchor) > off + ln or lm < off or lm + window > off + ln:
+                continue
+            w = enc[lm:lm + window]
+            site, spam = (w[:20], w[20:]) if strand == "+" else (revcomp(w[plen:]), revcomp(w[:plen]))
+            if not all((IUPAC_BITS[p] >> CODE[b]) & 1 for b, p in zip(spam, pam)):
+                continue
+            mm = {i + 1 for i in range(20) if guide[i] != site[i]}
+            g0 = lm if strand == "+" else lm + plen
+            nsym = "A" if strand == "+" else "T"
+            for i in range(20):
+                if (isn[g0 + i] if strand == "+" else isn[g0 + 19 - i]) and guide[i] == nsym:
+                    mm.add(i + 1)
+            pos = sorted(mm)
+            score = sum(1.2 ** p for p in pos) if pos else 0.0
+            if score <= cutoff:
+                out.add((lm, strand, tuple(pos), score))
+    return out
+
+
+def put(contigs, k, pos, text):
+    name, s = contigs[k]
+    contigs[k] = (name, s[:pos] + text + s[pos + len(text):])
+
+
+def near(rng, guide, n_hot, n_cold, c0=8):
+    s = list(guide[:20])
+    idx = list(rng.choice(c0, size=n_hot, replace=False)) + \
+        list(c0 + rng.choice(20 - c0, size=n_cold, replace=False))
+    for i in idx:
+        s[i] = "ACGT"[("ACGT".index(s[i]) + int(rng.integers(1, 4))) % 4]
+    return "".join(s)
+
+
+def fast_case(seed, lengths=(20, 20, 21, 20, 21, 20)):
+    rng = np.random.default_rng(seed)
+    contigs = random_contigs(rng, [int(x) for x in rng.integers(3000, 7000, size=4)], n_frac=0.02)
+    contigs = [(f"chr{i + 1}", s) for i, (_, s) in enumerate(contigs)]
+    pams = ["NGG", "NG", "NNGRRT", "NGG", "NGA", "NAG"]
+    guides = []
+    for i, gl in enumerate(lengths):
+        seq = "".join(rng.choice(list("ACGT"), size=gl))
+        pam = pams[i % len(pams)]
+        guides.append((seq, pam))
+        for _ in range(10):
+            site = near(rng, seq, int(rng.integers(0, 4)), int(rng.random() < 0.2))
+            pseq = "".join(rng.choice(list(IUPAC[c])) for c in pam)
+            plant(rng, contigs, site + seq[20:], pseq, 0, "+" if rng.random() < 0.5 else "-")
+    # N inside planted sites, at hot and cold positions
+    for i in range(4):
+        seq, pam = guides[i]
+        site = list(near(rng, seq, 1, 0))
+        site[int(rng.integers(0, 20))] = "N"
+        plant(rng, contigs, "".join(site), "".join(rng.choice(list(IUPAC[c])) for c in pam), 0, "+")
+    # tails: '+' window ending the genome, '-' window starting the last contig
+    seq, pam = guides[0]
+    tail = seq[:20] + "AGG"
+    put(contigs, len(contigs) - 1, len(contigs[-1][1]) - len(tail), tail)
+    put(contigs, len(contigs) - 1, 0, revcomp(near(rng, seq, 1, 0) + "TGG"))
+    # across the junction of chr1 and chr2
+    j = near(rng, seq, 1, 0) + "CGG"
+    put(contigs, 0, len(contigs[0][1]) - 10, j[:10])
+    put(contigs, 1, 0, j[10:])
+    return contigs, guides
+
+
+def product_candidates(finder, contigs, guides, cutoff):
+    """{guide index: {(leftmost, strand, positions, score)}} from the fast finder."""
+    ext = finder._ext
+    inputs = [GuideInput(f"g{i}", s, p, "3prime", 1, 20, "ABE") for i, (s, p) in enumerate(guides)]
+    keys, _ = ext._deduplicate_guides(inputs)
+    h = finder._candidates(keys, cutoff)
+    offs, lens = ext.chrom_offsets, ext.chrom_lengths
+    out = {i: set() for i in range(len(keys))}
+    for kk, st, pos, mm, sc in zip(h.key.tolist(), h.strand.tolist(), h.pos.tolist(),
+                                   h.mm_out.tolist(), h.score.tolist()):
+        window = 20 + len(keys[kk].pam)
+        if st == 0:
+            lm = pos
+        else:
+            c = int(np.searchsorted(offs, pos, side="right") - 1)
+            lm = int(offs[c] + lens[c] - (pos - offs[c]) - window)
+        pl = tuple(i + 1 for i in range(20) if (mm >> i) & 1)
+        out[kk].add((lm, "+-"[st], pl, sc))
+    return [out[i] for i in range(len(keys))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("cutoff", [5.0, 3.0])
+def test_candidates_equal_literal_bruteforce(gpu, seed, cutoff):
+    contigs, guides = fast_case(seed)
+    genome = synth.genome_from_strings(contigs)
+    f = make_finder(mode="fast", genome=genome, search_cutoff=cutoff)
+    got = product_candidates(f, contigs, guides, cutoff)
+    total = 0
+    for (seq, pam), cand in zip(guides, got):
+        want = brute_candidates(contigs, seq, pam, cutoff)
+        assert cand == want, (seq, pam)
+        total += len(want)
+    assert total >= 10
+
+
+@pytest.mark.gpu
+def test_21nt_minus_anchor_is_shifted(gpu):
+    """For 21-nt guides the reference's '-' anchor is rc(guide)[:12], the rc
+    of bases 10-21: a '-' site exists only where those equal bases 9-20 (a
+    run), and then the brute force and the product agree."""
+    rng = np.random.default_rng(8)
+    contigs = random_contigs(rng, [4000, 3000], n_frac=0.0)
+    contigs = [(f"chr{i + 1}", s) for i, (_, s) in enumerate(contigs)]
+    run = "".join(rng.choice(list("ACGT"), size=8)) + "C" * 13        # bases 9-21 all C
+    plain = "".join(rng.choice(list("ACGT"), size=21))
+    for seq in (run, plain):
+        for strand in "+-":
+            for _ in range(3):
+                plant(rng, contigs, near(rng, seq, 1, 0), "AGG", 0, strand)
+    genome = synth.genome_from_strings(contigs)
+    f = make_finder(mode="fast", genome=genome)
+    guides = [(run, "NGG"), (plain, "NGG")]
+    got = product_candidates(f, contigs, guides, 5.0)
+    want = [brute_candidates(contigs, s, p, 5.0) for s, p in guides]
+    assert got == want
+    assert any(st == "-" for _, st, _, _ in want[0])
+    assert not any(st == "-" for _, st, _, _ in want[1])
+
+
+def table_guides(rng, guides, contigs):
+    out = []
+    for i, (seq, pam) in enumerate(guides):
+        if len(seq) != 20:
+            continue
+        et, ws, we = [("ABE", 4, 8), ("CBE", 3, 9), ("ABE", 1, 20)][i % 3]
+        out.append(GuideInput(f"g{i}", seq, pam, "3prime", ws, we, et,
+                              base_editor=["ABE8e", None][i % 2]))
+    # an input sharing its id and sequence with another editor, and one with an on-target
+    out.append(GuideInput(out[0].guide_id, out[0].guide_seq, out[0].pam, "3prime", 1, 20, "CBE"))
+    out.append(GuideInput("ot", out[1].guide_seq, out[1].pam, "3prime", 1, 20, "ABE",
+                          on_target_chrom="chr3", on_target_pos=101, on_target_strand="+"))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("return_all,thr,cutoff", [(False, None, 5.0), (True, 5.0, 5.0),
+                                                   (False, 4.0, 3.0)])
+def test_table_equals_oracle(gpu, seed, return_all, thr, cutoff):
+    rng = np.random.default_rng(50 + seed)
+    contigs, guides = fast_case(seed, lengths=(20, 20, 20, 20, 20, 20))
+    inputs = table_guides(rng, guides, contigs)
+    # make the on-target input's site real: plant it at chr3:101 '+'
+    seq, pam = inputs[-1].guide_seq, inputs[-1].pam
+    put(contigs, 2, 100, seq + "".join(rng.choice(list(IUPAC[c])) for c in pam))
+    genome = synth.genome_from_strings(contigs)
+    f = make_finder(mode="fast", genome=genome, search_cutoff=cutoff, n_top_offtargets=4)
+    got = f.search(guides=inputs, return_all=return_all, count_score_threshold=thr, verbose=False)
+    want = FastFinderOracle(genome, search_cutoff=cutoff, n_top_offtargets=4).search(
+        inputs, return_all=return_all, count_score_threshold=thr)
+    assert len(want) > 5
+    pd.testing.assert_frame_equal(got, want)
