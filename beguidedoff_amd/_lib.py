@@ -20,7 +20,13 @@ HIT_DTYPE = np.dtype([
     ("guide", "<u4"), ("pos", "<u4"), ("x0", "<u4"), ("x1", "<u4"), ("mm_mask", "<u4"),
     ("strand", "u1"), ("n_mm", "u1"), ("editable", "u1"), ("pad", "u1"), ("score", "<f8"),
 ], align=True)
+FAST_ANCHOR_DTYPE = np.dtype([
+    ("key_lo", "<u4"), ("key_hi", "<u4"), ("a0", "<u4"), ("a1", "<u4"), ("g0", "<u4"),
+    ("g1", "<u4"), ("n_code", "<u4"), ("pam_nibs", "<u4"), ("guide", "<u4"), ("strand", "u1"),
+    ("pam_len", "u1"), ("cold_first", "u1"), ("pad", "u1"),
+], align=True)
 assert GUIDE_DESC_DTYPE.itemsize == 24 and HIT_DTYPE.itemsize == 32
+assert FAST_ANCHOR_DTYPE.itemsize == 40
 
 
 class Tuning(C.Structure):
@@ -71,6 +77,11 @@ _SIGNATURES = {
                                    C.POINTER(C.c_int64)]),
     "bgo_encode_genome": (C.c_int, [C.c_int32, _P, C.c_int64, _P, _P, C.c_int32, _P, _P, _P,
                                     C.POINTER(C.c_float)]),
+    "bgo_fast_index_create": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(_P)]),
+    "bgo_fast_index_destroy": (C.c_int, [_P]),
+    "bgo_fast_index_bytes": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "bgo_fast_index_query": (C.c_int, [_P, _P, C.c_int32, C.c_double, _P, _P, C.c_int64,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_float)]),
 }
 
 _lock = threading.Lock()

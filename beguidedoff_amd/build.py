@@ -19,7 +19,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["bgo_lib.hip", "bgo_plain.hip", "bgo_batched.hip", "bgo_encode.hip"]
+HIP_SOURCES = ["bgo_lib.hip", "bgo_plain.hip", "bgo_batched.hip", "bgo_encode.hip",
+               "bgo_fastidx.hip"]
 HOST_SOURCES = ["bgo_synth.c"]
 
 HIP_LIB = PKG / "libbgo_hip.so"

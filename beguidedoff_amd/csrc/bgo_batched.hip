@@ -34,17 +34,6 @@
 
 #include "bgo_common.h"
 
-struct bgo_genome {
-  int device = 0;
-  int64_t total_bp = 0;
-  int64_t n_words = 0;
-  uint64_t n_blocks = 0;  // 32-base blocks, including zero padding
-  int n_strands = 0;
-  uint2* d_planes[2] = {nullptr, nullptr};
-  uint32_t* d_nwords[2] = {nullptr, nullptr};
-  int64_t bytes = 0;
-};
-
 namespace bgo {
 namespace bk {
 

@@ -11,6 +11,20 @@
 
 #include "bgo.h"
 
+// A genome resident on one device: both strands as bit planes (block b: .x
+// bit j = bit 0 of base 32b + j's code, .y = bit 1) and little-endian N words
+// (one per block), zero padded past the genome.
+struct bgo_genome {
+  int device = 0;
+  int64_t total_bp = 0;
+  int64_t n_words = 0;
+  uint64_t n_blocks = 0;  // 32-base blocks, including zero padding
+  int n_strands = 0;
+  uint2* d_planes[2] = {nullptr, nullptr};
+  uint32_t* d_nwords[2] = {nullptr, nullptr};
+  int64_t bytes = 0;
+};
+
 namespace bgo {
 
 // ----------------------------------------------------------------- errors

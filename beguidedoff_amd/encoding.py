@@ -27,6 +27,26 @@ def encode_dna(seq: str) -> np.ndarray:
     return _ENC[raw]
 
 
+_RC_ENC = np.zeros(256, dtype=np.uint8)  # code of a character's complement (others -> 0)
+for _c, _v in (("A", 3), ("C", 2), ("G", 1), ("T", 0)):
+    _RC_ENC[ord(_c)] = _v
+
+
+def _char_rows(seqs: List[str], length: int) -> np.ndarray:
+    raw = np.frombuffer("".join(seqs).upper().encode("latin-1", "replace"), dtype=np.uint8)
+    return raw.reshape(len(seqs), length)
+
+
+def encode_dna_rows(seqs: List[str], length: int) -> np.ndarray:
+    """Equal-length DNA strings -> uint8[n, length] codes, as encode_dna row by row."""
+    return _ENC[_char_rows(seqs, length)]
+
+
+def reverse_complement_rows(seqs: List[str], length: int) -> np.ndarray:
+    """uint8[n, length] codes of reverse_complement_seq(s) for equal-length strings."""
+    return _RC_ENC[_char_rows(seqs, length)[:, ::-1]]
+
+
 def decode_dna(encoded: np.ndarray) -> str:
     """uint8 codes (0..3) -> DNA string."""
     codes = np.asarray(encoded, dtype=np.uint8)

@@ -21,6 +21,14 @@ The host then applies the anchor, N and score rules exactly:
   genome's end are scanned (the extensive mode's 26-position tail guard is
   lowered to 20 + the shortest PAM).
 
+Two engines find the candidates, with identical results:
+* "index" (default): an anchor index of the forward genome built in HBM
+  (``bgo_fast_index_*``: every position keyed by its 12 bases, grouped by key
+  with a counting sort on the GPU); a query reads only the anchors'
+  occurrences and checks their windows on the GPU;
+* "scan": the batched kernel at the cutoff's mismatch budget, filtered on the
+  host.
+
 The table is the extensive finder's (same 12 columns, sort and top-N).
 Differences from the reference, by construction: the anchor occurrence cap
 (`max_hits_per_anchor`, first hits in suffix-array order) is not applied, so
@@ -29,7 +37,9 @@ than 10,000 genome occurrences.
 """
 from __future__ import annotations
 
+import ctypes as C
 import logging
+import time
 from typing import List, Optional, Tuple
 
 import numpy as np
@@ -37,8 +47,11 @@ import pandas as pd
 
 from .finder import (_AllHits, _W20, _LOGGER, _PROFILES_TSV, assemble_results, beguidedoff,
                      cctop_scores)
-from .encoding import encode_dna
+from . import _lib
+from .encoding import encode_dna_rows, reverse_complement_rows
 from .io import load_guides_from_tsv
+from .pam import encode_pam_mask
+from .scoring import cctop_table
 from .types import GuideInput
 
 _GUIDE_LENGTH = 20
@@ -91,6 +104,110 @@ def _n_bits20(nmask_le: Optional[np.ndarray], pos: np.ndarray) -> np.ndarray:
     return ((v >> (pos & 7).astype(np.uint64)) & np.uint64(_W20)).astype(np.uint32)
 
 
+def anchor_table(scan_guides, cutoff: float) -> np.ndarray:
+    """Two anchors per scan key ('+' then '-'), as bgo_fast_anchor records
+    (fast/finder.py:179-189): '+' reads guide[c0:20] on the forward genome,
+    '-' reads rc(guide)[:m] there, i.e. guide[len - m:] on the reverse
+    strand's site positions c0..19.  The index key range holds the keys whose
+    first min(m, 12) bases are the anchor's.  Built per guide length with
+    table lookups over all keys at once."""
+    c0, m = cold_region(cutoff)
+    k = min(m, 12)
+    n_keys = len(scan_guides)
+    seqs = [sg.guide_seq.upper() for sg in scan_guides]
+    lens = np.array([len(x) for x in seqs], dtype=np.int64)
+    if n_keys and int(lens.min()) < _GUIDE_LENGTH:
+        raise ValueError(f"fast mode needs guides of at least {_GUIDE_LENGTH} nt")
+    plus = np.zeros((n_keys, _GUIDE_LENGTH), dtype=np.uint32)
+    minus = np.zeros((n_keys, _GUIDE_LENGTH), dtype=np.uint32)
+    first = np.zeros((n_keys, 2, k), dtype=np.uint32)  # the anchors' first k forward bases
+    for L in np.unique(lens).tolist():
+        idx = np.flatnonzero(lens == L)
+        group = [seqs[i] for i in idx]
+        fwd = encode_dna_rows(group, L).astype(np.uint32)
+        plus[idx] = fwd[:, :_GUIDE_LENGTH]
+        minus[idx, :c0] = fwd[:, :c0]
+        minus[idx, c0:] = fwd[:, L - m:]
+        first[idx, 0] = fwd[:, c0:c0 + k]
+        first[idx, 1] = reverse_complement_rows(group, L)[:, :k]
+    pams = [sg.pam for sg in scan_guides]
+    nib_of = {}
+    for pam in set(pams):
+        pm = encode_pam_mask(pam)[: len(pam)].astype(np.uint32)
+        nib_of[pam] = int(np.bitwise_or.reduce(pm << (4 * np.arange(pm.size, dtype=np.uint32))))
+    sh = np.arange(_GUIDE_LENGTH, dtype=np.uint32)
+
+    def planes(c):
+        return (((c & 1) << sh).sum(axis=1).astype(np.uint32),
+                ((c >> 1) << sh).sum(axis=1).astype(np.uint32))
+
+    key_sh = (2 * (11 - np.arange(k))).astype(np.uint32)
+    lo = (first << key_sh).sum(axis=2).astype(np.uint32)  # [key, strand]
+    out = np.zeros(2 * n_keys, dtype=_lib.FAST_ANCHOR_DTYPE)
+    g0, g1 = planes(plus)
+    for s, site in ((0, plus), (1, minus)):
+        r = out[s::2]
+        r["key_lo"] = lo[:, s]
+        r["key_hi"] = lo[:, s] + np.uint32(1 << (2 * (12 - k)))
+        r["a0"], r["a1"] = planes(site)
+        r["g0"], r["g1"] = g0, g1
+        r["n_code"] = ((plus == (0 if s == 0 else 3)).astype(np.uint32) << sh).sum(axis=1)
+        r["pam_nibs"] = np.array([nib_of[p] for p in pams], dtype=np.uint32)
+        r["guide"] = np.arange(n_keys, dtype=np.uint32)
+        r["strand"] = s
+        r["pam_len"] = np.array([len(p) for p in pams], dtype=np.uint8)
+        r["cold_first"] = c0
+    return out
+
+
+class FastIndexGPU:
+    """Anchor index of a resident genome in HBM (bgo_fast_index_*)."""
+
+    def __init__(self, device_genome, chrom_offsets, chrom_lengths):
+        self._genome = device_genome  # must outlive the index
+        offs = np.ascontiguousarray(chrom_offsets, dtype=np.int64)
+        lens = np.ascontiguousarray(chrom_lengths, dtype=np.int64)
+        h = C.c_void_p()
+        t0 = time.perf_counter()
+        _lib.check(_lib.load().bgo_fast_index_create(device_genome.handle, _lib.ptr(offs),
+                                                     _lib.ptr(lens), int(offs.size), C.byref(h)))
+        self.build_s = time.perf_counter() - t0
+        self._h = h
+        self.score_table = np.array(cctop_table(), dtype=np.float64)
+
+    @property
+    def nbytes(self) -> int:
+        n = C.c_int64(0)
+        _lib.check(_lib.load().bgo_fast_index_bytes(self._h, C.byref(n)))
+        return int(n.value)
+
+    def query(self, anchors: np.ndarray, cutoff: float, capacity: int = 1 << 16):
+        """Candidate records of the anchors and the kernel time (ms); re-runs
+        once with an exact-size buffer when the first was too small."""
+        anchors = np.ascontiguousarray(anchors, dtype=_lib.FAST_ANCHOR_DTYPE)
+        while True:
+            out = np.empty(capacity, dtype=_lib.HIT_DTYPE)
+            n, ms = C.c_int64(0), C.c_float(0.0)
+            _lib.check(_lib.load().bgo_fast_index_query(
+                self._h, _lib.ptr(anchors), int(anchors.size), float(cutoff),
+                _lib.ptr(self.score_table), _lib.ptr(out), int(out.size), C.byref(n),
+                C.byref(ms)))
+            if n.value <= out.size:
+                return out[: n.value], float(ms.value)
+            capacity = int(n.value)
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.load().bgo_fast_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class beguidedofffast:
     """Fast (anchored, score-cutoff) search with the reference's constructor
     and ``search`` arguments (fast/finder.py:90-128, 303-441).
@@ -100,14 +217,16 @@ class beguidedofffast:
     preloaded extensive finder (``beguidedoff_amd.beguidedoff(...)``) whose
     device-resident genome is shared, as the reference's server passes a
     preloaded index handle (fast/finder.py:114-119).  ``genome``, ``device``
-    and ``tuning`` are as for the extensive finder.
+    and ``tuning`` are as for the extensive finder.  ``engine`` is "index"
+    (default: an anchor index in HBM, built here) or "scan" (no index).
     """
 
     def __init__(self, index_path: Optional[str] = None, search_cutoff: float = 5.0,
                  scoring_method: str = "CCTOP", n_top_offtargets: int = 3,
                  max_hits_per_anchor: int = 10000, verbose: bool = False,
                  index: Optional[beguidedoff] = None, genome: Optional[dict] = None,
-                 device: Optional[str] = None, tuning: Optional[dict] = None):
+                 device: Optional[str] = None, tuning: Optional[dict] = None,
+                 engine: str = "index"):
         if scoring_method.upper() != "CCTOP":
             raise ValueError("fast mode only supports scoring_method='CCTOP'")
         self.index_path = str(index_path) if index_path is not None else ""
@@ -125,6 +244,13 @@ class beguidedofffast:
                                     device=device, tuning=tuning,
                                     n_top_offtargets=self.n_top_offtargets)
         self._ext.n_top_offtargets = self.n_top_offtargets
+        if engine not in ("index", "scan"):
+            raise ValueError(f"engine must be 'index' or 'scan', got {engine!r}")
+        self.engine = engine
+        self._index: Optional[FastIndexGPU] = None
+        if engine == "index":
+            self._index = FastIndexGPU(self._ext._device_genome(), self._ext.chrom_offsets,
+                                       self._ext.chrom_lengths)
 
     @property
     def total_bp(self) -> int:
@@ -137,10 +263,32 @@ class beguidedofffast:
     def _candidates(self, scan_guides, cutoff: float) -> _AllHits:
         """Fast-mode candidates of every scan key, as flat hits whose score,
         mismatch mask and count include the N positions."""
+        if self._index is not None:
+            return self._candidates_index(scan_guides, cutoff)
+        return self._candidates_scan(scan_guides, cutoff)
+
+    def _candidates_index(self, scan_guides, cutoff: float) -> _AllHits:
+        anchors = anchor_table(scan_guides, cutoff)
+        h, self.last_kernel_ms = self._index.query(anchors, cutoff)
+        key = h["guide"].astype(np.int64)
+        x0, x1 = h["x0"].astype(np.uint32), h["x1"].astype(np.uint32)
+        g0 = anchors["g0"][0::2][key] if key.size else np.zeros(0, np.uint32)
+        g1 = anchors["g1"][0::2][key] if key.size else np.zeros(0, np.uint32)
+        mm2 = ((x0 ^ g0) | (x1 ^ g1)) & np.uint32(_W20)
+        return _AllHits(key, h["strand"].astype(np.int8), h["pos"].astype(np.int64),
+                        h["n_mm"].astype(np.int64), mm2, h["score"].astype(np.float64), x0, x1,
+                        h["mm_mask"].astype(np.uint32))
+
+    def close(self) -> None:
+        if self._index is not None:
+            self._index.close()
+            self._index = None
+
+    def _candidates_scan(self, scan_guides, cutoff: float) -> _AllHits:
         from .batched import GuideSpec, scan
 
         ext = self._ext
-        c0, m = cold_region(cutoff)
+        c0, _ = cold_region(cutoff)
         budget = mismatch_budget(cutoff)
         plen_k = np.array([len(sg.pam) for sg in scan_guides], dtype=np.int64)
         specs = [GuideSpec(sg.guide_seq, sg.pam, budget) for sg in scan_guides]
@@ -160,22 +308,10 @@ class beguidedofffast:
 
         # anchors on the site's cold positions c0..19, per key and strand
         cold = np.uint32(((1 << (_GUIDE_LENGTH - c0)) - 1) << c0)
-        anchor = np.zeros((len(scan_guides), 2, 2), dtype=np.uint32)  # [key, strand, plane]
-        n_code = np.zeros((len(scan_guides), 2), dtype=np.uint32)      # guide == N code
-        for i, sg in enumerate(scan_guides):
-            g = encode_dna(sg.guide_seq.upper())
-            plus = np.zeros(_GUIDE_LENGTH, dtype=np.uint8)
-            plus[: min(g.size, _GUIDE_LENGTH)] = g[:_GUIDE_LENGTH]
-            minus = plus.copy()
-            minus[c0:] = g[g.size - m:]               # rc(guide)[:m] read on the site
-            anchor[i, 0] = _planes(plus)
-            anchor[i, 1] = _planes(minus)
-            sh = np.arange(_GUIDE_LENGTH, dtype=np.uint32)
-            n_code[i, 0] = int(((plus == 0).astype(np.uint32) << sh).sum())
-            n_code[i, 1] = int(((plus == 3).astype(np.uint32) << sh).sum())
+        anchors = anchor_table(scan_guides, cutoff).reshape(len(scan_guides), 2)
         s_idx = strand.astype(np.int64)
-        a0, a1 = anchor[key, s_idx, 0], anchor[key, s_idx, 1]
-        anchor_ok = (((x0 ^ a0) | (x1 ^ a1)) & cold) == 0
+        anc = anchors[key, s_idx]
+        anchor_ok = (((x0 ^ anc["a0"]) | (x1 ^ anc["a1"])) & cold) == 0
 
         # N positions that count as mismatches (fast/finder.py:227-257)
         nbits = np.zeros(pos.size, dtype=np.uint32)
@@ -183,10 +319,10 @@ class beguidedofffast:
             sel = strand == s
             if sel.any():
                 nbits[sel] = _n_bits20(nm, pos[sel])
-        mm = mm2 | (nbits & n_code[key, s_idx])
+        mm = mm2 | (nbits & anc["n_code"])
         score = cctop_scores(mm)
         keep = inside & anchor_ok & (score <= cutoff)
-        n_mm = np.array([bin(int(v)).count("1") for v in mm[keep]], dtype=np.int64)
+        n_mm = np.bitwise_count(mm[keep]).astype(np.int64)
         return _AllHits(key[keep], strand[keep], pos[keep], n_mm, mm2[keep], score[keep],
                         x0[keep], x1[keep], mm[keep])
 

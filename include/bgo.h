@@ -245,6 +245,52 @@ int bgo_scan_batched(bgo_genome* g, const bgo_guide_desc* guides,
                      const bgo_tuning* tuning, bgo_hit* out,
                      int64_t out_capacity, int64_t* n_hits_out);
 
+/* -------------------------------------------------------- fast-mode index
+ * Anchor index of a resident genome for the fast mode; replaces the
+ * reference's FM-index (fast/fmindex.py, fast/finder.py:149-159).  Every
+ * forward position is keyed by its 12 bases (2-bit codes, first base in bits
+ * 22-23, N read as A); positions are grouped by key on the GPU.  The index
+ * keeps a pointer to `g`, which must outlive it. */
+typedef struct bgo_fast_index bgo_fast_index;
+
+/* One anchor of one guide (fast/finder.py:186-189): its occurrences are the
+ * positions with keys in [key_lo, key_hi); each occurrence's window is
+ * checked on the GPU against the whole cold block (site positions
+ * cold_first..19 equal to a0/a1), the PAM, and score <= cutoff over the
+ * 2-bit mismatches against g0/g1 plus the N positions in n_code. */
+typedef struct bgo_fast_anchor {
+  uint32_t key_lo, key_hi;  /* key range of the anchor's first bases */
+  uint32_t a0, a1;          /* anchor planes on site positions cold_first..19 */
+  uint32_t g0, g1;          /* guide planes, site positions 0..19 */
+  uint32_t n_code;          /* site positions whose guide base is the strand's N code */
+  uint32_t pam_nibs;        /* allowed-base nibble of PAM position i at bits 4i */
+  uint32_t guide;           /* caller's guide index, copied to the hit record */
+  uint8_t strand;           /* 0 '+', 1 '-' */
+  uint8_t pam_len;          /* 1..6 */
+  uint8_t cold_first;       /* first cold site position (0-based) */
+  uint8_t pad;
+} bgo_fast_anchor;          /* 40 bytes */
+
+/* Build the index on g's device (contigs must tile the genome). */
+int bgo_fast_index_create(bgo_genome* g, const int64_t* contig_offsets,
+                          const int64_t* contig_lengths, int32_t n_contigs,
+                          bgo_fast_index** out);
+int bgo_fast_index_destroy(bgo_fast_index* x);
+int bgo_fast_index_bytes(const bgo_fast_index* x, int64_t* bytes_out);
+
+/* Candidates of n_anchors anchors as bgo_hit records: pos and x0/x1 on the
+ * scanned strand (as the batched scan reports them), mm_mask = 2-bit
+ * mismatches plus N positions, n_mm its popcount, score its CCTOP score
+ * (score_table[p] = 1.2**p, summed in ascending p).  Returns the exact count
+ * in n_hits_out and stores min(count, out_capacity) records (order
+ * unspecified).  The index keeps its query buffers between calls (grown on
+ * demand, freed by bgo_fast_index_destroy), so queries on one index must not
+ * overlap. */
+int bgo_fast_index_query(bgo_fast_index* x, const bgo_fast_anchor* anchors,
+                         int32_t n_anchors, double cutoff, const double* score_table,
+                         bgo_hit* out, int64_t out_capacity, int64_t* n_hits_out,
+                         float* kernel_ms);
+
 /* ---------------------------------------------------------- genome encoder
  * FASTA sequence bytes -> genome artefact arrays on `device`; replaces the
  * per-base loops of genome_converter.convert (genome_converter.py:15-108).

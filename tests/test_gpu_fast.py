@@ -1,15 +1,18 @@
 """The GPU fast mode (beguidedoff_amd/fast.py) against the fast-mode oracles:
 the table of oracle/fast_finder_oracle.py (20-nt guides, over the C
 restatement of the anchored candidate search) and a literal brute force of
-fast/finder.py:163-277 (20- and 21-nt guides).  Genomes carry N runs, sites
-planted at both genome tails, across contig junctions and with N inside."""
+fast/finder.py:163-277 (20- and 21-nt guides), for both candidate engines (the
+HBM anchor index and the budget scan).  Genomes carry N runs, sites planted at
+both genome tails, across contig junctions and with N inside; one genome is
+built of long N and poly-A runs and tandem copies of a site, so index buckets
+hold thousands of positions and runs of one key."""
 import numpy as np
 import pandas as pd
 import pytest
 
 from beguidedoff_amd import beguidedoff as make_finder
 from beguidedoff_amd import synth
-from beguidedoff_amd.fast import cold_region, mismatch_budget
+from beguidedoff_amd.fast import FastIndexGPU, anchor_table, cold_region, mismatch_budget
 from beguidedoff_amd.types import GuideInput
 from brute import IUPAC, plant, random_contigs, revcomp
 from oracle.fast_finder_oracle import FastFinderOracle
@@ -26,6 +29,39 @@ def test_regions_and_budget_known_answers():
     assert [mismatch_budget(t) for t in (5.0, 4.368, 4.36, 3.0, 1.2, 1.19)] == [3, 3, 2, 2, 1, 0]
     with pytest.raises(ValueError, match="cold"):
         cold_region(40.0)
+
+
+def test_anchor_table_known_answers():
+    """Anchor records decoded back to strings: keys are the anchors' first
+    min(m, 12) bases (fast/finder.py:179-189), sites carry the anchor on
+    positions c0..19, N codes mark A ('+') and T ('-') guide bases."""
+    from beguidedoff_amd.finder import _ScanGuide
+
+    def bases(p0, p1, n=20):
+        return "".join("ACGT"[((p0 >> i) & 1) | (((p1 >> i) & 1) << 1)] for i in range(n))
+
+    def key(s):
+        return sum(CODE[c] << (2 * (11 - j)) for j, c in enumerate(s))
+
+    g20, g21 = "ACGTTGCAACGTTGCAAGGC", "TTGACCAGTACGGATCCATGA"
+    for cutoff, c0, m in ((5.0, 8, 12), (3.0, 6, 14), (7.0, 10, 10)):
+        a = anchor_table([_ScanGuide(g20, "NGRRT"), _ScanGuide(g21.lower(), "NG")], cutoff)
+        assert a["strand"].tolist() == [0, 1, 0, 1] and a["guide"].tolist() == [0, 0, 1, 1]
+        for r, g, st in zip(a, (g20, g20, g21, g21), (0, 1, 0, 1)):
+            anchor = g[c0:20] if st == 0 else revcomp(g)[:m]
+            k = min(m, 12)
+            assert (r["key_lo"], r["key_hi"]) == (key(anchor[:k]), key(anchor[:k]) + 4 ** (12 - k))
+            site = bases(r["a0"], r["a1"])
+            assert site[:c0] == g[:c0]
+            assert site[c0:] == (g[c0:20] if st == 0 else g[len(g) - m:])
+            assert bases(r["g0"], r["g1"]) == g[:20]
+            n_sym = "A" if st == 0 else "T"
+            assert r["n_code"] == sum(1 << i for i in range(20) if g[i] == n_sym)
+            assert r["cold_first"] == c0
+        assert a["pam_nibs"].tolist() == [0x8554F, 0x8554F, 0x4F, 0x4F]
+        assert a["pam_len"].tolist() == [5, 5, 2, 2]
+    with pytest.raises(ValueError, match="at least 20"):
+        anchor_table([_ScanGuide("ACGT" * 4, "NGG")], 5.0)
 
 
 def brute_candidates(contigs, guide, pam, cutoff):
@@ -135,12 +171,15 @@ def product_candidates(finder, contigs, guides, cutoff):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["index", "scan"])
 @pytest.mark.parametrize("seed", range(4))
-@pytest.mark.parametrize("cutoff", [5.0, 3.0])
-def test_candidates_equal_literal_bruteforce(gpu, seed, cutoff):
+@pytest.mark.parametrize("cutoff", [5.0, 3.0, 7.0])
+def test_candidates_equal_literal_bruteforce(gpu, seed, cutoff, engine):
+    """Cutoffs 5 and 3 have a cold block of 12 and 14 bases (one index key,
+    then a key plus two checked bases), 7 one of 10 (a range of 16 keys)."""
     contigs, guides = fast_case(seed)
     genome = synth.genome_from_strings(contigs)
-    f = make_finder(mode="fast", genome=genome, search_cutoff=cutoff)
+    f = make_finder(mode="fast", genome=genome, search_cutoff=cutoff, engine=engine)
     got = product_candidates(f, contigs, guides, cutoff)
     total = 0
     for (seq, pam), cand in zip(guides, got):
@@ -151,7 +190,8 @@ def test_candidates_equal_literal_bruteforce(gpu, seed, cutoff):
 
 
 @pytest.mark.gpu
-def test_21nt_minus_anchor_is_shifted(gpu):
+@pytest.mark.parametrize("engine", ["index", "scan"])
+def test_21nt_minus_anchor_is_shifted(gpu, engine):
     """For 21-nt guides the reference's '-' anchor is rc(guide)[:12], the rc
     of bases 10-21: a '-' site exists only where those equal bases 9-20 (a
     run), and then the brute force and the product agree."""
@@ -165,7 +205,7 @@ def test_21nt_minus_anchor_is_shifted(gpu):
             for _ in range(3):
                 plant(rng, contigs, near(rng, seq, 1, 0), "AGG", 0, strand)
     genome = synth.genome_from_strings(contigs)
-    f = make_finder(mode="fast", genome=genome)
+    f = make_finder(mode="fast", genome=genome, engine=engine)
     guides = [(run, "NGG"), (plain, "NGG")]
     got = product_candidates(f, contigs, guides, 5.0)
     want = [brute_candidates(contigs, s, p, 5.0) for s, p in guides]
@@ -207,3 +247,62 @@ def test_table_equals_oracle(gpu, seed, return_all, thr, cutoff):
         inputs, return_all=return_all, count_score_threshold=thr)
     assert len(want) > 5
     pd.testing.assert_frame_equal(got, want)
+
+
+def repeat_case(seed):
+    """Contigs of random sequence, a 6 kb N run, a 4 kb poly-A and poly-T run
+    and 400 tandem copies of a near-site (bucket sizes past a block's 256
+    threads, many positions of one key in a wave)."""
+    rng = np.random.default_rng(seed)
+    unit_guide = "".join(rng.choice(list("ACGT"), size=20))
+    unit = near(rng, unit_guide, 1, 0) + "TGG"
+    contigs = random_contigs(rng, [5000, 4000, 3000], n_frac=0.0)
+    contigs = [("chr1", contigs[0][1] + "N" * 6000 + contigs[1][1]),
+               ("chr2", "A" * 4000 + unit * 400 + "T" * 4000),
+               ("chr3", contigs[2][1])]
+    guides = [(unit_guide, "NGG"), ("A" * 20, "NGG"), ("T" * 20, "NGG"), ("A" * 19 + "G", "NNN"),
+              ("T" * 19 + "C", "NN"), (unit_guide[::-1], "NG")]
+    for seq, pam in guides[3:]:
+        for strand in "+-":
+            plant(rng, contigs, near(rng, seq, 1, 0), "".join(rng.choice(list(IUPAC[c])) for c in pam),
+                  0, strand)
+    return contigs, guides
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cutoff", [5.0, 7.0])
+def test_index_on_repeats_and_n_runs(gpu, cutoff):
+    contigs, guides = repeat_case(3)
+    genome = synth.genome_from_strings(contigs)
+    f = make_finder(mode="fast", genome=genome, search_cutoff=cutoff)
+    got = product_candidates(f, contigs, guides, cutoff)
+    want = [brute_candidates(contigs, s, p, cutoff) for s, p in guides]
+    assert got == want
+    assert len(want[0]) == 400 and sum(len(w) for w in want) > 400
+
+
+@pytest.mark.gpu
+def test_index_query_grows_its_buffer_and_sizes(gpu):
+    contigs, guides = repeat_case(4)
+    genome = synth.genome_from_strings(contigs)
+    f = make_finder(mode="fast", genome=genome)
+    ext = f._ext
+    keys, _ = ext._deduplicate_guides(
+        [GuideInput(f"g{i}", s, p, "3prime", 1, 20, "ABE") for i, (s, p) in enumerate(guides)])
+    anchors = anchor_table(keys, 5.0)
+    big, _ = f._index.query(anchors, 5.0)
+    small, ms = f._index.query(anchors, 5.0, capacity=1)
+    assert len(big) >= 400 and ms >= 0.0
+
+    def rows(h):
+        return sorted(zip(h["guide"].tolist(), h["strand"].tolist(), h["pos"].tolist(),
+                          h["mm_mask"].tolist(), h["score"].tolist()))
+    assert rows(big) == rows(small)
+    n = sum(len(s) for _, s in contigs)
+    assert f._index.nbytes == 4 * ((1 << 24) + 1) + 4 * n + 16 * len(contigs)
+    assert f._index.build_s > 0.0
+    # a second index over the same resident genome answers the same
+    other = FastIndexGPU(ext._device_genome(), ext.chrom_offsets, ext.chrom_lengths)
+    assert rows(other.query(anchors, 5.0)[0]) == rows(big)
+    other.close()
+    f.close()
