@@ -230,14 +230,17 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 //    are +1 (bit 0) / -1 (bit 1); guide elements 60..63 hold kmax - 15 and
 //    window ones +1, so a product is 4 * (kmax - mismatches).
 //  * ENGINE 3, fp4 e2m1 (v_mfma_scale_f32_32x32x64_f8f6f4, K = 64 in one
-//    step, twice the int8 rate): window elements are 0.5 * bit, guide
-//    elements -1 (bit 0) / +1 (bit 1).  With window bits a and guide bits b,
-//    sum(a (2b - 1)) = |b| - 2 mm and |b| = 2 x (non-A guide bases), so the
-//    code part is (non-A guide bases) - mm; guide elements 60..63 sum to
-//    budget - (non-A guide bases) against window elements of 1.0, and the
-//    f32 product is exactly budget - mm.  Element 32 g + i of lane group g
-//    sits in dword i % 4 at nibble i / 4, so a window's operand dword d is
-//    (code word >> d) & 0x11111111.
+//    step, twice the int8 rate): every code element contributes
+//    0.5 * bit * (+1 if the guide's bit is 1, else -1).  With window bits a
+//    and guide bits b, sum(a (2b - 1)) / 2 = |b| / 2 - mm and |b| = 2 x
+//    (non-A guide bases), so the code part is (non-A guide bases) - mm; four
+//    bias elements add budget - (non-A guide bases), and the f32 product is
+//    exactly budget - mm.  Element i of lane group g sits in dword i % 4 at
+//    nibble i / 4; the window's element is its bit at nibble bit i % 4 (value
+//    0.5, 1.0 or 2.0; i % 4 = 3 is shifted down to 0.5), the guide's weight
+//    +-1 / value, stored doubled (+-2, +-1, +-0.5) under a B scale of 2^-1.
+//    The bias elements are 2.0 in the window, so the guide's four stored
+//    values sum to the bias.
 // Padding guides get a negative bias, so they never reach the exact path.
 // Lane layouts (checked on gfx950): A/B row or column = lane % 32; lane group
 // lane / 32 holds half of K and the hardware pairs equal (group, slot) of A
@@ -255,10 +258,13 @@ __device__ inline v4i expand16(uint32_t bits) {
   return r;
 }
 
-// fp4 A operand dword d of a 32-bit code word: nibble k holds 0.5 * (bit
-// 4k + d of the word), OR-ed with the pad pattern.
+// fp4 A operand dword d of a 32-bit code word c, OR-ed with the pad
+// pattern: nibble k holds bit 4k + d of c, in place for d = 0, 1, 2 (e2m1
+// 0.5, 1.0 and 2.0) and shifted to the nibble's low bit for d = 3 (0.5), so
+// three of the four dwords take one AND each.
 __device__ inline uint32_t fp4_dword(uint32_t c, uint32_t d, uint32_t pad) {
-  return ((c >> d) & 0x11111111u) | pad;
+  if (d == 3) return ((c >> 3) & 0x11111111u) | pad;
+  return (c & (0x11111111u << d)) | pad;
 }
 
 // Exact check of the (window, guide) pairs of one guide tile whose
@@ -339,8 +345,9 @@ __device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const ui
 __device__ inline v16i fp4_mma(const v8i& A, v4i b) {
   const v16f zero = {};
   const v8i B = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+  // B scale 2^-1 (E8M0 126): guide weights are stored doubled
   return __builtin_bit_cast(
-      v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, zero, 4, 4, 0, 127, 0, 127));
+      v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, zero, 4, 4, 0, 127, 0, 126));
 }
 
 // AND of a product's 16 elements as 32-bit patterns: the sign bit is clear
@@ -378,22 +385,20 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
                                          uint32_t lane) {
   constexpr uint32_t kAT = ring_flush(ENGINE) / 32;  // A tiles
   const uint32_t m = lane & 31u, g = lane >> 5;
-  uint32_t lo[kAT], hi[kAT];  // code elements 0..31 and 32..63 of window rows 32 i + m
-#pragma unroll
-  for (uint32_t tile = 0; tile < kAT; ++tile) {
-    const uint32_t widx = 32u * tile + m;
-    uint2 e = make_uint2(0u, 0u);
-    if (widx < n) e = L[widx];
-    const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
-    lo[tile] = W1 | (W0 << 20);
-    hi[tile] = (W0 >> 12) | ((W0 ^ W1) << 8);
-  }
   if constexpr (ENGINE == 3) {
+    // Lane group g's 32 code bits of window row 32 tile + m: the b0 (g = 0)
+    // or b1 (g = 1) plane of the 20 bases, then bits 0..11 (g = 0) or
+    // 12..19 (g = 1) of b0^b1 at bits 20 and up; bits 28..31 of group 1
+    // are the bias elements.
+    const uint32_t xmask = g ? 0xFF000u : 0xFFFu, xshift = g ? 8u : 20u;
+    const uint32_t pad = g ? 0x40000000u : 0u;  // bias elements = 2.0
     v8i A[4];
-    const uint32_t pad = g ? 0x20000000u : 0u;  // elements 60..63 = 1.0
 #pragma unroll
     for (int tile = 0; tile < 4; ++tile) {
-      const uint32_t c = g ? hi[tile] : lo[tile];
+      const uint32_t widx = 32u * tile + m;
+      uint2 e = make_uint2(0u, 0u);
+      if (widx < n) e = L[widx];
+      const uint32_t c = ((g ? e.y : e.x) & kWin) | (((e.x ^ e.y) & xmask) << xshift);
       A[tile] = v8i{static_cast<int>(fp4_dword(c, 0, pad)), static_cast<int>(fp4_dword(c, 1, pad)),
                     static_cast<int>(fp4_dword(c, 2, pad)), static_cast<int>(fp4_dword(c, 3, pad)),
                     0, 0, 0, 0};
@@ -464,6 +469,16 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
       b1 = btile(ti + 3);
     }
   } else {
+    uint32_t lo[kAT], hi[kAT];  // code elements 0..31 and 32..63 of window rows 32 i + m
+#pragma unroll
+    for (uint32_t tile = 0; tile < kAT; ++tile) {
+      const uint32_t widx = 32u * tile + m;
+      uint2 e = make_uint2(0u, 0u);
+      if (widx < n) e = L[widx];
+      const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
+      lo[tile] = W1 | (W0 << 20);
+      hi[tile] = (W0 >> 12) | ((W0 ^ W1) << 8);
+    }
     const bool two = n > 32;
     v4i A[2][2];
 #pragma unroll
@@ -1096,9 +1111,17 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       for (size_t gi = F.g_begin; gi < F.g_end; ++gi) {
         uint8_t nib[64] = {};
         if (gk[gi] >= 0) {
-          const uint64_t g0 = g01[2 * gi], g1 = g01[2 * gi + 1];
-          const uint64_t bits = g1 | (g0 << 20) | ((g0 ^ g1) << 40);
-          for (int k = 0; k < 60; ++k) nib[k] = ((bits >> k) & 1u) ? 0x2 : 0xA;  // +1 / -1
+          const uint32_t g0 = g01[2 * gi] & bk::kWin, g1 = g01[2 * gi + 1] & bk::kWin;
+          const uint32_t x = g0 ^ g1;
+          // the window code words of lane groups 0 and 1 (bias bits clear)
+          const uint32_t code[2] = {g0 | ((x & 0xFFFu) << 20), g1 | ((x & 0xFF000u) << 8)};
+          // stored weight +-1 / (window element value) for i % 4 = 0, 1, 2, 3
+          static const uint8_t pos_w[4] = {0x4, 0x2, 0x1, 0x4};  // 2, 1, 0.5, 2
+          for (int k = 0; k < 60; ++k) {
+            const int grp = k / 32, i = k % 32;
+            const uint8_t w = pos_w[i % 4];
+            nib[k] = ((code[grp] >> i) & 1u) ? w : static_cast<uint8_t>(w | 0x8);
+          }
           const int bias = gk[gi] - __builtin_popcount(static_cast<uint32_t>(g0 | g1) & bk::kWin);
           if (!fp4_sum4(bias, nib + 60)) return fail(BGO_ERR_ARG, "guide bias %d out of range", bias);
         } else {
