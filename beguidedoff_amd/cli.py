@@ -6,8 +6,9 @@ subcommands and options (beguidedoff/cli.py:75-126):
     python -m beguidedoff_amd.cli verify --encoded G.npz --fasta G.fa
 
 ``search --mode fast`` runs the fast (anchored, score-cutoff) search on the GPU
-over an encoded genome, which is its index: ``build-index`` writes that
-artefact (the same file as ``encode-genome``), and ``--index`` names it.
+over an encoded genome: ``build-index --output hg38.fmi`` writes that artefact
+under the given name (plus the reference's ``.meta`` and ``.nmask``
+sidecars), and ``--index hg38.fmi`` names it.
 """
 from __future__ import annotations
 
@@ -27,8 +28,11 @@ def _encode_genome(args) -> int:
 
 
 def _build_index(args) -> int:
-    """cli.py:20-24: the fast mode's index is the encoded genome artefact."""
-    genome_converter.convert(args.fasta, args.output, verbose=True)
+    """cli.py:20-24: the fast mode's index files: the encoded genome artefact
+    at --output, with the reference's .meta and .nmask sidecars."""
+    from .fast import build_index
+
+    build_index(fasta_path=args.fasta, output_path=args.output, verbose=True)
     return 0
 
 
@@ -83,10 +87,9 @@ def build_parser() -> argparse.ArgumentParser:
     enc.add_argument("--device", type=int, default=0, help="HIP device for --engine hip")
     enc.set_defaults(func=_encode_genome)
 
-    idx = sub.add_parser("build-index",
-                         help="Build the fast mode's index (the encoded genome .npz)")
+    idx = sub.add_parser("build-index", help="Build the fast mode's index files")
     idx.add_argument("--fasta", required=True, help="Input FASTA file")
-    idx.add_argument("--output", required=True, help="Output .npz path")
+    idx.add_argument("--output", required=True, help="Output index path (e.g. hg38.fmi)")
     idx.set_defaults(func=_build_index)
 
     s = sub.add_parser("search", help="Search for off-targets")

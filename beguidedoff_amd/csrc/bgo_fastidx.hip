@@ -34,6 +34,12 @@ struct bgo_fast_index {
   int64_t hits_cap = 0;
   double* d_tab = nullptr;                // [21]
   unsigned long long* d_count = nullptr;  // [1]
+  uint32_t* d_chunks = nullptr;           // [anchors] occurrence chunks per anchor
+  int64_t chunks_cap = 0;
+  uint64_t* d_first = nullptr;            // [anchors + 1] first chunk of each anchor
+  int64_t first_cap = 0;
+  std::vector<uint32_t> h_chunks;
+  std::vector<uint64_t> h_first;
 };
 
 namespace bgo {
@@ -42,6 +48,7 @@ namespace fi {
 constexpr uint32_t kBuckets = 1u << 24;
 constexpr uint32_t kScanChunk = 4096;  // keys per block in the scan (256 threads x 16)
 constexpr uint32_t kWin = 0xFFFFFu;
+constexpr uint32_t kQueryChunk = 1024;  // occurrences per workgroup of the query
 
 // 12 bits -> even bit positions (bit i -> bit 2i).
 __device__ inline uint32_t spread12(uint32_t x) {
@@ -170,6 +177,17 @@ __global__ void scatter_kernel(const uint2* __restrict__ planes, uint64_t n_pos,
   }
 }
 
+// Occurrence chunks of each anchor's key range.
+__global__ void chunk_count_kernel(const bgo_fast_anchor* __restrict__ anchors, int32_t n,
+                                   const uint32_t* __restrict__ offsets,
+                                   uint32_t* __restrict__ chunks) {
+  const int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) {
+    const uint32_t occ = offsets[anchors[i].key_hi] - offsets[anchors[i].key_lo];
+    chunks[i] = (occ + kQueryChunk - 1) / kQueryChunk;
+  }
+}
+
 struct QueryArgs {
   const uint2* planes0;
   const uint2* planes1;
@@ -180,6 +198,9 @@ struct QueryArgs {
   const int64_t* contigs;  // offsets, then lengths
   int32_t n_contigs;
   const bgo_fast_anchor* anchors;
+  int32_t n_anchors;
+  const uint64_t* first;  // [n_anchors + 1] first chunk of each anchor, then the total
+  uint64_t n_chunks;
   double cutoff;
   const double* score_tab;  // [21]
   bgo_hit* hits;
@@ -196,57 +217,74 @@ __device__ inline int32_t contig_of(const int64_t* __restrict__ offs, int32_t n,
   return lo;
 }
 
-// One block per anchor; its threads stride over the anchor's occurrences.
+// The anchor owning chunk c: the last a with first[a] <= c.
+__device__ inline int32_t anchor_of(const uint64_t* __restrict__ first, int32_t n, uint64_t c) {
+  int32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= c) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One workgroup per chunk of kQueryChunk occurrences of one anchor (grid-
+// stride over the chunks); its threads take one occurrence each.
 __global__ void query_kernel(QueryArgs q) {
-  const bgo_fast_anchor A = q.anchors[blockIdx.x];
-  const uint32_t begin = q.offsets[A.key_lo], end = q.offsets[A.key_hi];
-  const uint32_t plen = A.pam_len, window = 20 + plen;
-  const uint32_t cold = (kWin >> A.cold_first) << A.cold_first;
-  const uint2* __restrict__ planes = A.strand ? q.planes1 : q.planes0;
-  const uint32_t* __restrict__ nwords = A.strand ? q.nwords1 : q.nwords0;
-  const int64_t* offs = q.contigs;
-  const int64_t* lens = q.contigs + q.n_contigs;
-  for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    const int64_t gpos = q.pos[i];
-    const int64_t left = A.strand ? gpos - plen : gpos - A.cold_first;
-    if (left < 0) continue;
-    const int32_t k = contig_of(offs, q.n_contigs, left);
-    const int64_t c_off = offs[k], c_end = offs[k] + lens[k];
-    if (left < c_off || left + window > c_end) continue;
-    // the window on the scanned strand (the reverse complement keeps contig order)
-    const int64_t s = A.strand ? c_off + (c_end - left) - window : left;
-    const uint64_t b = static_cast<uint64_t>(s) >> 5;
-    const uint32_t sh = static_cast<uint32_t>(s & 31);
-    const uint2 w0 = planes[b], w1 = planes[b + 1];
-    const uint32_t x0 = funnel(w1.x, w0.x, sh) & 0x3FFFFFFu;
-    const uint32_t x1 = funnel(w1.y, w0.y, sh) & 0x3FFFFFFu;
-    if (((x0 ^ A.a0) | (x1 ^ A.a1)) & cold) continue;
-    bool pam_ok = true;
-    for (uint32_t j = 0; j < plen; ++j) {
-      const uint32_t code = ((x0 >> (20 + j)) & 1u) | (((x1 >> (20 + j)) & 1u) << 1);
-      pam_ok &= ((A.pam_nibs >> (4 * j + code)) & 1u) != 0;
-    }
-    if (!pam_ok) continue;
-    // a genome without N carries no N words (null)
-    const uint32_t nb = nwords ? funnel(nwords[b + 1], nwords[b], sh) & kWin : 0u;
-    const uint32_t mm = ((((x0 ^ A.g0) | (x1 ^ A.g1)) & kWin) | (nb & A.n_code));
-    double score = 0.0;
-    for (uint32_t m = mm; m; m &= m - 1) score += q.score_tab[__builtin_ctz(m) + 1];
-    if (!(score <= q.cutoff)) continue;
-    const unsigned long long idx = atomicAdd(q.n_hits, 1ull);
-    if (idx < q.cap) {
-      bgo_hit h;
-      h.guide = A.guide;
-      h.pos = static_cast<uint32_t>(s);
-      h.x0 = x0;
-      h.x1 = x1;
-      h.mm_mask = mm;
-      h.strand = A.strand;
-      h.n_mm = static_cast<uint8_t>(__popc(mm));
-      h.editable = 0;
-      h.pad = 0;
-      h.score = score;
-      q.hits[idx] = h;
+  for (uint64_t c = blockIdx.x; c < q.n_chunks; c += gridDim.x) {
+    const int32_t ai = anchor_of(q.first, q.n_anchors, c);
+    const bgo_fast_anchor A = q.anchors[ai];
+    const uint32_t key_begin = q.offsets[A.key_lo], key_end = q.offsets[A.key_hi];
+    const uint64_t from = key_begin + (c - q.first[ai]) * kQueryChunk;
+    const uint32_t begin = static_cast<uint32_t>(from);
+    const uint32_t end = static_cast<uint32_t>(min<uint64_t>(from + kQueryChunk, key_end));
+    const uint32_t plen = A.pam_len, window = 20 + plen;
+    const uint32_t cold = (kWin >> A.cold_first) << A.cold_first;
+    const uint2* __restrict__ planes = A.strand ? q.planes1 : q.planes0;
+    const uint32_t* __restrict__ nwords = A.strand ? q.nwords1 : q.nwords0;
+    const int64_t* offs = q.contigs;
+    const int64_t* lens = q.contigs + q.n_contigs;
+    for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+      const int64_t gpos = q.pos[i];
+      const int64_t left = A.strand ? gpos - plen : gpos - A.cold_first;
+      if (left < 0) continue;
+      const int32_t k = contig_of(offs, q.n_contigs, left);
+      const int64_t c_off = offs[k], c_end = offs[k] + lens[k];
+      if (left < c_off || left + window > c_end) continue;
+      // the window on the scanned strand (the reverse complement keeps contig order)
+      const int64_t s = A.strand ? c_off + (c_end - left) - window : left;
+      const uint64_t b = static_cast<uint64_t>(s) >> 5;
+      const uint32_t sh = static_cast<uint32_t>(s & 31);
+      const uint2 w0 = planes[b], w1 = planes[b + 1];
+      const uint32_t x0 = funnel(w1.x, w0.x, sh) & 0x3FFFFFFu;
+      const uint32_t x1 = funnel(w1.y, w0.y, sh) & 0x3FFFFFFu;
+      if (((x0 ^ A.a0) | (x1 ^ A.a1)) & cold) continue;
+      bool pam_ok = true;
+      for (uint32_t j = 0; j < plen; ++j) {
+        const uint32_t code = ((x0 >> (20 + j)) & 1u) | (((x1 >> (20 + j)) & 1u) << 1);
+        pam_ok &= ((A.pam_nibs >> (4 * j + code)) & 1u) != 0;
+      }
+      if (!pam_ok) continue;
+      // a genome without N carries no N words (null)
+      const uint32_t nb = nwords ? funnel(nwords[b + 1], nwords[b], sh) & kWin : 0u;
+      const uint32_t mm = ((((x0 ^ A.g0) | (x1 ^ A.g1)) & kWin) | (nb & A.n_code));
+      double score = 0.0;
+      for (uint32_t m = mm; m; m &= m - 1) score += q.score_tab[__builtin_ctz(m) + 1];
+      if (!(score <= q.cutoff)) continue;
+      const unsigned long long idx = atomicAdd(q.n_hits, 1ull);
+      if (idx < q.cap) {
+        bgo_hit h;
+        h.guide = A.guide;
+        h.pos = static_cast<uint32_t>(s);
+        h.x0 = x0;
+        h.x1 = x1;
+        h.mm_mask = mm;
+        h.strand = A.strand;
+        h.n_mm = static_cast<uint8_t>(__popc(mm));
+        h.editable = 0;
+        h.pad = 0;
+        h.score = score;
+        q.hits[idx] = h;
+      }
     }
   }
 }
@@ -271,6 +309,11 @@ hipError_t reserve(T** buf, int64_t* cap, int64_t need) {
 }
 
 void free_query_buffers(bgo_fast_index* X) {
+  if (X->d_chunks) (void)hipFree(X->d_chunks);
+  if (X->d_first) (void)hipFree(X->d_first);
+  X->d_chunks = nullptr;
+  X->d_first = nullptr;
+  X->chunks_cap = X->first_cap = 0;
   if (X->d_anchors) (void)hipFree(X->d_anchors);
   if (X->d_hits) (void)hipFree(X->d_hits);
   if (X->d_tab) (void)hipFree(X->d_tab);
@@ -397,19 +440,40 @@ extern "C" int bgo_fast_index_query(bgo_fast_index* X, const bgo_fast_anchor* an
   BGO_HIP(hipSetDevice(X->device));
   BGO_HIP(fi::reserve(&X->d_anchors, &X->anchors_cap, n_anchors));
   BGO_HIP(fi::reserve(&X->d_hits, &X->hits_cap, std::max<int64_t>(out_capacity, 1)));
+  BGO_HIP(fi::reserve(&X->d_chunks, &X->chunks_cap, n_anchors));
+  BGO_HIP(fi::reserve(&X->d_first, &X->first_cap, n_anchors + 1));
   BGO_HIP(hipMemcpy(X->d_anchors, anchors, n_anchors * sizeof(bgo_fast_anchor),
                     hipMemcpyHostToDevice));
   BGO_HIP(hipMemcpy(X->d_tab, score_table, 21 * sizeof(double), hipMemcpyHostToDevice));
   BGO_HIP(hipMemset(X->d_count, 0, sizeof(unsigned long long)));
+  // chunks of each anchor's occurrences, and their prefix on the host
+  hipLaunchKernelGGL(fi::chunk_count_kernel, dim3((n_anchors + 255) / 256), dim3(256), 0, 0,
+                     X->d_anchors, n_anchors, X->d_offsets, X->d_chunks);
+  BGO_HIP(hipGetLastError());
+  X->h_chunks.resize(n_anchors);
+  X->h_first.resize(n_anchors + 1);
+  BGO_HIP(hipMemcpy(X->h_chunks.data(), X->d_chunks, n_anchors * sizeof(uint32_t),
+                    hipMemcpyDeviceToHost));
+  uint64_t total = 0;
+  for (int32_t i = 0; i < n_anchors; ++i) {
+    X->h_first[i] = total;
+    total += X->h_chunks[i];
+  }
+  X->h_first[n_anchors] = total;
+  BGO_HIP(hipMemcpy(X->d_first, X->h_first.data(), (n_anchors + 1) * sizeof(uint64_t),
+                    hipMemcpyHostToDevice));
   fi::QueryArgs q{g->d_planes[0], g->d_planes[1], g->d_nwords[0], g->d_nwords[1],
-                  X->d_offsets, X->d_pos, X->d_contigs, X->n_contigs, X->d_anchors, cutoff,
-                  X->d_tab, X->d_hits, X->d_count, static_cast<uint64_t>(out_capacity)};
+                  X->d_offsets, X->d_pos, X->d_contigs, X->n_contigs, X->d_anchors, n_anchors,
+                  X->d_first, total, cutoff, X->d_tab, X->d_hits, X->d_count,
+                  static_cast<uint64_t>(out_capacity)};
+  // at most 2^20 workgroups; more chunks are taken grid-stride
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(total, 1u << 20)));
   hipEvent_t e0, e1;
   BGO_HIP(hipEventCreate(&e0));
   BGO_HIP(hipEventCreate(&e1));
   hipError_t e = hipEventRecord(e0, 0);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(fi::query_kernel, dim3(static_cast<uint32_t>(n_anchors)), dim3(256), 0, 0, q);
+    hipLaunchKernelGGL(fi::query_kernel, dim3(grid), dim3(256), 0, 0, q);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipEventRecord(e1, 0);

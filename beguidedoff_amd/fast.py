@@ -48,6 +48,7 @@ import pandas as pd
 from .finder import (_AllHits, _W20, _LOGGER, _PROFILES_TSV, assemble_results, beguidedoff,
                      cctop_scores)
 from . import _lib
+from . import genome_converter
 from .encoding import encode_dna_rows, reverse_complement_rows
 from .io import load_guides_from_tsv
 from .pam import encode_pam_mask
@@ -55,6 +56,52 @@ from .scoring import cctop_table
 from .types import GuideInput
 
 _GUIDE_LENGTH = 20
+_META_HEADER = "#beguidedoff-fmindex\t1"
+
+
+def _n_runs(seq: str) -> np.ndarray:
+    """Half-open [start, end) runs of 'N' (after upper-casing) as an (n, 2) array."""
+    raw = np.frombuffer(seq.upper().encode("latin-1", "replace"), dtype=np.uint8)
+    isn = np.concatenate(([0], (raw == ord("N")).astype(np.int8), [0]))
+    d = np.diff(isn)
+    return np.stack([np.flatnonzero(d == 1), np.flatnonzero(d == -1)], axis=1)
+
+
+def build_index(fasta_path: str, output_path: str, verbose: bool = False,
+                engine: str = "numpy", device: int = 0) -> None:
+    """The fast mode's index files (fast/fmindex.py:63-128), written under the
+    reference's names:
+
+    * ``output_path``: the encoded genome artefact (the genome_converter
+      arrays, npz content at exactly this path), which the fast finder loads;
+      the anchor index itself is built in HBM from it at start-up;
+    * ``output_path + ".meta"``: the chromosome table in the reference's
+      format (csrc/fmindex_binding.cpp:47-56), a header line then
+      ``name<TAB>length<TAB>offset`` with offsets in the reference's index
+      text, where one separator symbol follows every contig but the last;
+    * ``output_path + ".nmask"``: the N runs (upper-cased 'N') as half-open
+      ``start<TAB>end`` lines in the same coordinates (fast/fmindex.py:94-111).
+    """
+    contigs = list(genome_converter.read_fasta(fasta_path))
+    arrays = genome_converter.encode_contigs(contigs, include_rc=True, include_n_mask=True,
+                                             engine=engine, device=device)
+    with open(output_path, "wb") as fh:
+        np.savez_compressed(fh, **arrays)
+    offset = 0
+    with open(str(output_path) + ".meta", "w") as meta, \
+            open(str(output_path) + ".nmask", "w") as nmask:
+        meta.write(_META_HEADER + "\n")
+        for k, (name, seq) in enumerate(contigs):
+            if k:
+                offset += 1  # the separator between contigs
+            meta.write(f"{name}\t{len(seq)}\t{offset}\n")
+            for start, end in _n_runs(seq).tolist():
+                nmask.write(f"{offset + start}\t{offset + end}\n")
+            offset += len(seq)
+            if verbose:
+                _LOGGER.info("  indexed %s (%d bp)", name, len(seq))
+    if verbose:
+        _LOGGER.info("Wrote %s (+ .meta, .nmask)", output_path)
 
 
 def cold_region(cutoff: float) -> Tuple[int, int]:

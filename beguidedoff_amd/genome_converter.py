@@ -132,7 +132,9 @@ def convert(fasta_path: str, output_path: str, include_rc: bool = True,
                             engine=engine, device=device)
     np.savez_compressed(output_path, **arrays)
     if verbose:
-        mb = Path(output_path).stat().st_size / (1024 * 1024)
+        # numpy appends ".npz" to a name without it
+        written = output_path if str(output_path).endswith(".npz") else f"{output_path}.npz"
+        mb = Path(written).stat().st_size / (1024 * 1024)
         print(f"  Total genome: {int(arrays['total_bp']):,} bp; saved {output_path} ({mb:.1f} MB)")
 
 
@@ -148,7 +150,11 @@ def load_genome(encoded_path: str) -> dict:
 
     Loaded without pickle: ``chrom_names`` is a unicode array.
     """
-    data = np.load(encoded_path, allow_pickle=False)
+    try:
+        data = np.load(encoded_path, allow_pickle=False)
+    except ValueError as exc:
+        raise ValueError(f"{encoded_path} is not an encoded genome artefact (an .npz written by "
+                         "encode-genome or build-index)") from exc
     out = {
         "genome_forward": data["genome_forward"],
         "chrom_names": data["chrom_names"],

@@ -250,14 +250,14 @@ def test_table_equals_oracle(gpu, seed, return_all, thr, cutoff):
 
 
 def repeat_case(seed):
-    """Contigs of random sequence, a 6 kb N run, a 4 kb poly-A and poly-T run
+    """Contigs of random sequence, a 60 kb N run, a 4 kb poly-A and poly-T run
     and 400 tandem copies of a near-site (bucket sizes past a block's 256
     threads, many positions of one key in a wave)."""
     rng = np.random.default_rng(seed)
     unit_guide = "".join(rng.choice(list("ACGT"), size=20))
     unit = near(rng, unit_guide, 1, 0) + "TGG"
     contigs = random_contigs(rng, [5000, 4000, 3000], n_frac=0.0)
-    contigs = [("chr1", contigs[0][1] + "N" * 6000 + contigs[1][1]),
+    contigs = [("chr1", contigs[0][1] + "N" * 60000 + contigs[1][1]),
                ("chr2", "A" * 4000 + unit * 400 + "T" * 4000),
                ("chr3", contigs[2][1])]
     guides = [(unit_guide, "NGG"), ("A" * 20, "NGG"), ("T" * 20, "NGG"), ("A" * 19 + "G", "NNN"),
