@@ -139,6 +139,41 @@ def encoder_probe(workload: str) -> dict:
             "call_s_with_pcie": round(wall, 3), "outputs_equal_generator": exact}
 
 
+def fast_probe(workload: str, cutoff: float = 5.0, reps: int = 5) -> dict:
+    """The GPU fast mode's anchor index (bgo_fast_index_*) on a generated
+    genome: build time and bytes, then `reps` queries of every 20-nt guide of
+    the workload (kernel time from HIP events, and the call end to end)."""
+    from beguidedoff_amd.fast import FastIndexGPU, anchor_table
+    from beguidedoff_amd.batched import DeviceGenome
+    from beguidedoff_amd.finder import _ScanGuide
+    from beguidedoff_amd import _lib, synth
+
+    _lib.init(0)
+    w = synth.workload(workload)
+    g = w.genome
+    keys = list(dict.fromkeys(_ScanGuide(x.guide_seq, x.pam) for x in w.guides
+                              if len(x.guide_seq) == 20))
+    dg = DeviceGenome(g, g["n_mask_le_fwd"], g["n_mask_le_rc"])
+    ix = FastIndexGPU(dg, g["chrom_offsets"], g["chrom_lengths"])
+    log(f"fast probe: index of {int(g['total_bp'])} positions built in {ix.build_s:.3f} s")
+    anchors = anchor_table(keys, cutoff)
+    kms, walls, n = [], [], 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        hits, ms = ix.query(anchor_table(keys, cutoff), cutoff)
+        walls.append(time.perf_counter() - t0)
+        kms.append(ms)
+        n = int(hits.size)
+    out = {"metric": "fast_index", "workload": workload, "positions": int(g["total_bp"]),
+           "guides": len(keys), "anchors": int(anchors.size), "cutoff": cutoff,
+           "index_build_s": round(ix.build_s, 4), "index_bytes": ix.nbytes,
+           "query_kernel_ms": round(float(np.median(kms)), 4),
+           "query_call_ms": round(1e3 * float(np.median(walls)), 3), "candidates": n}
+    ix.close()
+    dg.close()
+    return out
+
+
 def cpu_baseline(workload, max_mm: int, target_s: float = 12.0) -> dict:
     from oracle.scan import scan_strand
     from beguidedoff_amd.encoding import encode_dna, encode_guide_64bit
@@ -357,6 +392,9 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration; "
                          "a process of its own: torch carries its own HIP runtime)")
+    ap.add_argument("--fast-probe", action="store_true",
+                    help="build the fast mode's anchor index on the workload's genome and time "
+                         "queries of its 20-nt guides, then exit")
     ap.add_argument("--encoder-probe", action="store_true",
                     help="only time the GPU genome encoder on the workload's genome text")
     args = ap.parse_args()
@@ -372,6 +410,9 @@ def main() -> int:
         return 0
     if args.encoder_probe:
         print(json.dumps(encoder_probe(args.workload)), flush=True)
+        return 0
+    if args.fast_probe:
+        print(json.dumps(fast_probe(args.workload)), flush=True)
         return 0
 
     from beguidedoff_amd import _lib, synth
