@@ -856,8 +856,10 @@ extern "C" int bgo_genome_create(const uint32_t* words_fwd, const uint32_t* word
       (nmask_bytes > 0 && words_rc && !nmask_rc))
     return fail(BGO_ERR_ARG, "bgo_genome_create: bad N mask");
   BGO_TRY(injected());
+  int device = 0;
+  BGO_HIP(hipGetDevice(&device));
   auto* G = new bgo_genome();
-  BGO_HIP(hipGetDevice(&G->device));
+  G->device = device;
   G->total_bp = total_bp;
   G->n_words = n_words;
   // +4 zero blocks: the kernel reads planes[w + 1] for w <= (total_bp + 63) / 32.

@@ -76,6 +76,20 @@ inline int injected() {
     if (rc_ != BGO_OK) return rc_; \
   } while (0)
 
+// Two timing events, destroyed with the object (also on early returns).
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  hipError_t create() {
+    hipError_t r = hipEventCreate(&e[0]);
+    if (r == hipSuccess) r = hipEventCreate(&e[1]);
+    return r;
+  }
+  ~EventPair() {
+    for (hipEvent_t ev : e)
+      if (ev) (void)hipEventDestroy(ev);
+  }
+};
+
 // ------------------------------------------------------------ device math
 
 // Gather bits 0, 2, 4, ..., 30 of x into bits 0..15.

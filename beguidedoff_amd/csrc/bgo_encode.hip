@@ -164,9 +164,9 @@ extern "C" int bgo_encode_genome(int32_t device, const uint8_t* seq, int64_t n_b
   BGO_HIP(hipMemcpy(offs, contig_offsets, n_contigs * 8, hipMemcpyHostToDevice));
   BGO_HIP(hipMemcpy(offs + n_contigs, contig_lengths, n_contigs * 8, hipMemcpyHostToDevice));
 
-  hipEvent_t e0, e1;
-  BGO_HIP(hipEventCreate(&e0));
-  BGO_HIP(hipEventCreate(&e1));
+  EventPair ev;
+  BGO_HIP(ev.create());
+  const hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   const dim3 grid(static_cast<uint32_t>((n_words + 255) / 256)), block(256);
   const auto* s = static_cast<const uint8_t*>(d_seq.p);
   hipError_t err = hipEventRecord(e0, 0);
@@ -185,8 +185,6 @@ extern "C" int bgo_encode_genome(int32_t device, const uint8_t* seq, int64_t n_b
   if (err == hipSuccess) err = hipEventSynchronize(e1);
   float ms = 0.f;
   if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   BGO_HIP(err);
   if (kernel_ms) *kernel_ms = ms;
   if (words_fwd) BGO_HIP(hipMemcpy(words_fwd, d_fwd.p, n_words * 4, hipMemcpyDeviceToHost));

@@ -40,6 +40,7 @@ struct bgo_fast_index {
   int64_t first_cap = 0;
   std::vector<uint32_t> h_chunks;
   std::vector<uint64_t> h_first;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // query kernel timing
 };
 
 namespace bgo {
@@ -314,6 +315,10 @@ void free_query_buffers(bgo_fast_index* X) {
   X->d_chunks = nullptr;
   X->d_first = nullptr;
   X->chunks_cap = X->first_cap = 0;
+  for (hipEvent_t& ev : X->ev) {
+    if (ev) (void)hipEventDestroy(ev);
+    ev = nullptr;
+  }
   if (X->d_anchors) (void)hipFree(X->d_anchors);
   if (X->d_hits) (void)hipFree(X->d_hits);
   if (X->d_tab) (void)hipFree(X->d_tab);
@@ -366,6 +371,8 @@ extern "C" int bgo_fast_index_create(bgo_genome* g, const int64_t* contig_offset
   if (e == hipSuccess) e = hipMalloc(&X->d_contigs, 2 * static_cast<size_t>(n_contigs) * 8);
   if (e == hipSuccess) e = hipMalloc(&X->d_tab, 21 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&X->d_count, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipEventCreate(&X->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&X->ev[1]);
   if (e == hipSuccess) e = hipMalloc(&counts.p, fi::kBuckets * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&cursor.p, fi::kBuckets * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&totals.p, (fi::kBuckets / fi::kScanChunk) * sizeof(uint32_t));
@@ -468,9 +475,7 @@ extern "C" int bgo_fast_index_query(bgo_fast_index* X, const bgo_fast_anchor* an
                   static_cast<uint64_t>(out_capacity)};
   // at most 2^20 workgroups; more chunks are taken grid-stride
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(total, 1u << 20)));
-  hipEvent_t e0, e1;
-  BGO_HIP(hipEventCreate(&e0));
-  BGO_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = X->ev[0], e1 = X->ev[1];
   hipError_t e = hipEventRecord(e0, 0);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(fi::query_kernel, dim3(grid), dim3(256), 0, 0, q);
@@ -480,8 +485,6 @@ extern "C" int bgo_fast_index_query(bgo_fast_index* X, const bgo_fast_anchor* an
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0.f;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   BGO_HIP(e);
   unsigned long long n = 0;
   BGO_HIP(hipMemcpy(&n, X->d_count, sizeof(n), hipMemcpyDeviceToHost));
