@@ -555,13 +555,14 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
       continue;
     }
     do {
-      // The entry is formed in every lane and stored by those with a window;
-      // pw & (pw - 1) leaves an empty word empty.
-      const uint32_t j = __builtin_ctz(pw | 0x80000000u);
-      const uint32_t slot = lanes_below(act, cnt);
-      const uint2 e = make_uint2((funnel(qb.x, qa.x, j) & kWin) | (x0 | (j << 20)),
-                                 (funnel(qb.y, qa.y, j) & kWin) | y0);
-      if (pw) L[slot] = e;
+      // Lanes with a window store it at their rank among them; pw & (pw - 1)
+      // leaves an empty word empty.
+      if (pw) {
+        const uint32_t j = static_cast<uint32_t>(__builtin_ctz(pw));
+        const uint32_t slot = lanes_below(act, cnt);
+        L[slot] = make_uint2((funnel(qb.x, qa.x, j) & kWin) | lshl_or(j, 20, x0),
+                             (funnel(qb.y, qa.y, j) & kWin) | y0);
+      }
       pw &= pw - 1;
       cnt += static_cast<uint32_t>(__popcll(act));
       act = ballot(pw != 0);

@@ -107,6 +107,14 @@ __device__ inline uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t sh) {
   return __builtin_amdgcn_alignbit(hi, lo, sh);
 }
 
+// (a << sh) | b as one v_lshl_or_b32 (the compiler otherwise re-associates
+// a constant-shifted OR into an OR followed by a shift).
+__device__ inline uint32_t lshl_or(uint32_t a, uint32_t sh, uint32_t b) {
+  uint32_t r;
+  asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "i"(sh), "v"(b));
+  return r;
+}
+
 __device__ inline uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
