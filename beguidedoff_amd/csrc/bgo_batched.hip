@@ -59,6 +59,7 @@ struct Args {
   uint32_t valid_begin, valid_end;
   uint32_t chunk, chunks_per_slot, n_items, n_fam;
   uint32_t n_slots, interleave;  // interleave: consecutive items alternate strands
+  uint32_t btab_bytes, pad_;     // size of the MFMA guide operand table
   unsigned long long hit_cap;
 };
 
@@ -406,11 +407,15 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
     const uint32_t nat = (n + 31) / 32;
     // Two operand registers, each reloaded two tiles ahead (no copies); the
     // table ends with four spare tiles for the look-ahead loads.
-    // tile base scalar, lane offset in a VGPR (saddr loads, no per-load VALU)
-    const char* __restrict__ bt = reinterpret_cast<const char*>(t.btab) + (F.g_begin / 32) * 1024;
+    // Guide tiles through a buffer resource: the tile offset in an SGPR, the
+    // lane offset in a VGPR, so a load needs no VALU address arithmetic.
+    const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int4*>(t.btab), 0, static_cast<int>(a.btab_bytes), 0x00020000);
+    const uint32_t fbase = (F.g_begin / 32) * 1024u;
     const uint32_t l16 = lane * 16u;
     auto btile = [&](uint32_t tile) {
-      return *reinterpret_cast<const v4i*>(bt + tile * 1024u + l16);
+      return __builtin_bit_cast(
+          v4i, __builtin_amdgcn_raw_buffer_load_b128(brs, l16, fbase + tile * 1024u, 0));
     };
     const uint32_t n_tiles = (F.g_end - F.g_begin) / 32;
     v4i b0 = btile(0), b1 = btile(1);
@@ -1305,6 +1310,7 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
     a.n_slots = n_slots;
     a.interleave = P->tune.reserved[0] ? 1u : 0u;
     a.n_fam = static_cast<uint32_t>(nf);
+    a.btab_bytes = static_cast<uint32_t>(P->h_btab.size());
     a.hit_cap = static_cast<unsigned long long>(P->hit_cap);
     hipLaunchKernelGGL(bk::kernel_for(nf, P->tune.engine), dim3(P->group_blocks[gi]), dim3(P->threads),
                        P->group_lds[gi], P->stream, a, P->d_fam + f0, P->d_g01, P->d_gk, P->d_gaux, P->d_gid,
