@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from .base_editing import EDITABLE_BASE, edit_window_mask
-from .encoding import encode_dna, encode_guide_64bit
+from .encoding import encode_dna_rows
 from .pam import encode_pam_mask
 from .scoring import cctop_table
 
@@ -38,19 +38,42 @@ class GuideSpec:
 
 
 def guide_descs(specs: Sequence[GuideSpec]) -> np.ndarray:
-    d = np.zeros(len(specs), dtype=_lib.GUIDE_DESC_DTYPE)
-    for i, s in enumerate(specs):
-        d["seq2bit"][i] = encode_guide_64bit(encode_dna(s.seq)) & ((1 << 40) - 1)
-        d["pam_mask"][i] = encode_pam_mask(s.pam)
-        d["pam_len"][i] = len(s.pam)
-        if not 0 <= int(s.max_mm) <= 20:
-            raise ValueError(f"max_mm must be in 0..20, got {s.max_mm}")
-        d["max_mm"][i] = int(s.max_mm)
-        target = EDITABLE_BASE.get(s.editor_type)
+    """bgo_guide_desc records: the first 20 bases packed 2 bits each (base i
+    at bits 2i), PAM nibbles and length, budget, editor and edit window.
+    Sequences are encoded per length group and PAMs, editors and windows once
+    per distinct value."""
+    n = len(specs)
+    d = np.zeros(n, dtype=_lib.GUIDE_DESC_DTYPE)
+    if n == 0:
+        return d
+    seqs = [s.seq for s in specs]
+    lens = np.array([len(x) for x in seqs], dtype=np.int64)
+    packed = np.zeros(n, dtype=np.uint64)
+    for L in np.unique(lens).tolist():
+        idx = np.flatnonzero(lens == L)
+        k = min(L, 20)
+        if k == 0:
+            continue
+        codes = encode_dna_rows([seqs[i] for i in idx], L)[:, :k].astype(np.uint64)
+        packed[idx] = (codes << (2 * np.arange(k, dtype=np.uint64))).sum(axis=1, dtype=np.uint64)
+    d["seq2bit"] = packed
+    mm = np.array([int(s.max_mm) for s in specs], dtype=np.int64)
+    bad = np.flatnonzero((mm < 0) | (mm > 20))
+    if bad.size:
+        raise ValueError(f"max_mm must be in 0..20, got {specs[int(bad[0])].max_mm}")
+    d["max_mm"] = mm
+    pam_of = {p: encode_pam_mask(p) for p in {s.pam for s in specs}}
+    d["pam_mask"] = np.stack([pam_of[s.pam] for s in specs])
+    d["pam_len"] = [len(s.pam) for s in specs]
+    editor_of = {}
+    for e in {s.editor_type for s in specs}:
+        target = EDITABLE_BASE.get(e)
         if target is None:
-            raise ValueError(f"Unknown editor_type: {s.editor_type}")
-        d["editor"][i] = _EDITOR_CODE[target]
-        d["edit_mask"][i] = edit_window_mask(*s.window)
+            raise ValueError(f"Unknown editor_type: {e}")
+        editor_of[e] = _EDITOR_CODE[target]
+    d["editor"] = [editor_of[s.editor_type] for s in specs]
+    window_of = {w: edit_window_mask(*w) for w in {tuple(s.window) for s in specs}}
+    d["edit_mask"] = [window_of[tuple(s.window)] for s in specs]
     return d
 
 

@@ -34,7 +34,7 @@ import pandas as pd
 
 from . import _lib
 from .base_editing import EDITABLE_BASE, edit_window_mask
-from .encoding import encode_dna, encode_guide_64bit
+from .encoding import encode_dna, encode_dna_rows, encode_guide_64bit
 from .genome_converter import load_genome
 from .io import OUTPUT_COLUMNS, load_guides_from_tsv
 from .pam import encode_pam_mask
@@ -500,6 +500,40 @@ def _mm_string(mask: int) -> str:
     return s
 
 
+def _guide_planes(seqs: List[str]) -> Tuple[np.ndarray, np.ndarray]:
+    """Bit planes (b0, b1) of the first 20 bases of each guide, base i at bit i."""
+    p0 = np.zeros(len(seqs), dtype=np.uint32)
+    p1 = np.zeros(len(seqs), dtype=np.uint32)
+    lens = np.array([len(x) for x in seqs], dtype=np.int64)
+    for L in np.unique(lens).tolist():
+        k = min(L, 20)
+        if k == 0:
+            continue
+        idx = np.flatnonzero(lens == L)
+        codes = encode_dna_rows([seqs[i] for i in idx], L)[:, :k].astype(np.uint32)
+        sh = np.arange(k, dtype=np.uint32)
+        p0[idx] = ((codes & 1) << sh).sum(axis=1, dtype=np.uint32)
+        p1[idx] = ((codes >> 1) << sh).sum(axis=1, dtype=np.uint32)
+    return p0, p1
+
+
+def _top_candidates(pid: np.ndarray, score: np.ndarray, n_ids: int, top: int) -> np.ndarray:
+    """Indices of the hits whose score is at most their id's top-th smallest
+    (all of an id's hits when it has fewer): every hit that can be among an
+    id's first `top` rows in (score, ...) order, ties at the cut included."""
+    if top <= 0 or pid.size == 0:
+        return np.zeros(0, dtype=np.int64)
+    o = np.argsort(score)  # ties in any order: only each id's cut value is used
+    p = pid[o]
+    o = o[np.argsort(p.astype(np.uint16) if n_ids <= 0xFFFF else p, kind="stable")]
+    per = np.bincount(pid, minlength=n_ids)
+    starts = np.cumsum(per) - per
+    has = per > 0
+    thr = np.full(n_ids, -np.inf)
+    thr[has] = score[o[starts[has] + np.minimum(per[has], top) - 1]]
+    return np.flatnonzero(score <= thr[pid])
+
+
 def cctop_scores(mm_mask: np.ndarray) -> np.ndarray:
     """sum(1.2 ** p) over set positions, accumulated in ascending p: the same
     IEEE additions as calculate_cctop_score (x + 0.0 leaves x unchanged)."""
@@ -589,13 +623,7 @@ def assemble_results(finder: "beguidedoff", guides: List[GuideInput],
             if sel.any():
                 codes[sel] = _codes_at(words, pos[sel], 26)
         x0, x1 = _bits26(codes & 1), _bits26(codes >> 1)
-    gp0 = np.zeros(len(scan_guides), dtype=np.uint32)
-    gp1 = np.zeros(len(scan_guides), dtype=np.uint32)
-    for i, sg in enumerate(scan_guides):
-        g = encode_dna(sg.guide_seq)[:20].astype(np.uint32)
-        sh = np.arange(g.size, dtype=np.uint32)
-        gp0[i] = int(((g & 1) << sh).sum())
-        gp1[i] = int(((g >> 1) << sh).sum())
+    gp0, gp1 = _guide_planes([sg.guide_seq for sg in scan_guides])
     mm_mask = ((x0 ^ gp0[key]) | (x1 ^ gp1[key])) & np.uint32(_W20)
     if h.score is not None:
         score = h.score[keep] if not keep.all() else h.score
@@ -650,7 +678,13 @@ def assemble_results(finder: "beguidedoff", guides: List[GuideInput],
     # (chromosome rank, position) and (strand, key, input) packed into int64 keys
     place = (chrom_rank[chrom] << 33) + local
     tie = (strand[hid].astype(np.int64) << 62) + (key[hid] << 31) + inp
-    order = np.lexsort((tie, place, score[hid], pid))
+    sc = score[hid]
+    if return_all:
+        order = np.lexsort((tie, place, sc, pid))
+    else:
+        # only each id's top-N scores (ties included) can be reported
+        order = _top_candidates(pid, sc, n_ids, finder.n_top_offtargets)
+        order = order[np.lexsort((tie[order], place[order], sc[order], pid[order]))]
     pid_s = pid[order]
     per_id = np.bincount(pid_s, minlength=n_ids)
     starts = np.cumsum(per_id) - per_id

@@ -80,3 +80,32 @@ def test_vector_cctop_equals_python_sum_bitwise():
     got = cctop_scores(masks)
     for m, s in zip(masks.tolist(), got.tolist()):
         assert s == calculate_cctop_score([i + 1 for i in range(20) if (m >> i) & 1])
+
+
+def test_top_candidates_keep_every_possible_top_row():
+    """The assembly orders only each id's hits scoring at most its top-th
+    smallest score (ties at the cut included): equal to a per-id brute force,
+    on scores with heavy ties and ids with fewer hits than the cut."""
+    from beguidedoff_amd.finder import _top_candidates
+
+    rng = np.random.default_rng(11)
+    for n_ids, n, top in ((7, 500, 3), (1, 40, 1), (300, 2000, 5), (70000, 5000, 2), (5, 0, 3)):
+        pid = rng.integers(0, n_ids, size=n)
+        score = np.round(rng.choice([0.0, 1.2, 1.44, 2.64, 4.2998, 5.0], size=n), 6)
+        got = sorted(_top_candidates(pid, score, n_ids, top).tolist())
+        want = []
+        for i in range(n):
+            mine = np.sort(score[pid == pid[i]])
+            if score[i] <= mine[min(top, mine.size) - 1]:
+                want.append(i)
+        assert got == want
+        assert _top_candidates(pid, score, n_ids, 0).size == 0
+
+
+@pytest.mark.parametrize("n_top", [1, 5])
+def test_epilogue_top_n_matches_oracle(n_top):
+    genome, guides = random_case(3)
+    got, want = replay(genome, guides, 5, return_all=False, count_score_threshold=5.0,
+                       n_top=n_top)
+    assert len(want) > 0
+    pd.testing.assert_frame_equal(got, want)
