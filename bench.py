@@ -339,7 +339,7 @@ def serving_leg(dg, pool, max_mm: int, rates, duration: float, seed: int = 0) ->
     gbp = dg.total_bp / 1e9
     out = {}
     for rate in rates:
-        with ScanServer(dg, n_segments=16, hit_budget=1 << 24, pipeline=2) as srv:
+        with ScanServer(dg, n_segments=4, hit_budget=1 << 24, pipeline=2) as srv:
             t0 = time.perf_counter()
             handles = []
             t_next = t0
