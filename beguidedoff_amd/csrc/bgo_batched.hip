@@ -41,6 +41,13 @@ constexpr int kMaxFam = 8;     // PAM families per kernel launch
 constexpr int kUnroll = 8;     // guides per compare step
 constexpr int kMaxBlock = 256; // threads per block (4 waves)
 constexpr uint32_t kWin = 0xFFFFFu;
+// Work items are handed out by kWorkCtrs counters per launch group, each in a
+// 256-byte slot of its own (kCtrStride 32-bit words): same-address device
+// atomics serialise at about 12 ns each on gfx950, so one counter for the
+// whole grid costs ~0.1 ms to start 8192 waves and bounds a scan to one item
+// per 12 ns.
+constexpr uint32_t kWorkCtrs = 16;
+constexpr uint32_t kCtrStride = 64;
 
 struct Fam {
   uint32_t nibs;            // allowed-base nibble of PAM position i at bits 4i..4i+3 (N past the PAM)
@@ -701,11 +708,25 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
   w.stage = reinterpret_cast<uint2*>(w.counts + kMaxFam);
   if (lane < kMaxFam) w.wstats[lane] = 0;
   wave_sync();
+  // Counter k hands out items k, k + kWorkCtrs, k + 2 kWorkCtrs, ...  Global
+  // wave g takes from counter (g + g / kWorkCtrs) % kWorkCtrs: each aligned
+  // group of kWorkCtrs waves covers every counter (the host launches at least
+  // one group), and the per-group rotation gives every counter waves on all
+  // XCDs (blocks are dealt to the XCDs round-robin), so a counter's items are
+  // balanced across the chip.  Each subset spreads evenly over the scanned
+  // range, so the subsets end together, and a wave exits after one failed
+  // take on its own counter: a launch costs two takes per wave, kWorkCtrs-fold
+  // fewer per address than with one counter.
+  const uint32_t gw = blockIdx.x * n_waves + wave;
+  const uint32_t k = __builtin_amdgcn_readfirstlane((gw + gw / kWorkCtrs) % kWorkCtrs);
+  unsigned int* const my_counter = work_counter + k * kCtrStride;
   for (;;) {
-    uint32_t item = 0;
-    if (lane == 0) item = atomicAdd(work_counter, 1u);
-    item = __builtin_amdgcn_readlane(item, 0);
-    if (item >= a.n_items) break;
+    uint32_t r = 0;
+    if (lane == 0) r = atomicAdd(my_counter, 1u);
+    r = __builtin_amdgcn_readlane(r, 0);
+    const uint64_t item64 = static_cast<uint64_t>(r) * kWorkCtrs + k;
+    if (item64 >= a.n_items) break;
+    const uint32_t item = static_cast<uint32_t>(item64);
     uint32_t slot, run;
     if (a.interleave) {
       slot = item % a.n_slots;
@@ -792,10 +813,12 @@ struct bgo_plan {
   uint32_t* d_gid = nullptr;
   uint32_t* d_gnm = nullptr;
   double* d_score = nullptr;
-  unsigned long long* d_counters = nullptr;  // [1 hit count][groups work counters][fam windows]
+  // [1 hit count][fam windows][to 256 B][groups x kWorkCtrs work counters, 256 B each]
+  unsigned long long* d_counters = nullptr;
   int8_t* d_btab = nullptr;                  // MFMA guide operand tiles (engine 2)
   std::vector<int8_t> h_btab;
   uint32_t n_counter_words = 0;
+  uint32_t work_word = 0;  // 64-bit word of the first work counter
   bgo_hit* d_hits = nullptr;
   bgo_hit* h_pinned = nullptr;  // page-locked copy of the records (fetch_pinned)
   int64_t pinned_cap = 0;
@@ -964,7 +987,12 @@ int plan_configure_launch(bgo_plan* P) {
     if (P->tune.blocks_per_cu > 0) occ = std::min(occ, P->tune.blocks_per_cu);
     P->n_cu = n_cu;
     min_occ = std::min(min_occ, occ);
-    P->group_blocks.push_back(occ * n_cu);
+    // Wave w takes items from work counter w % kWorkCtrs only, so the grid
+    // needs a wave for every counter (blocks past occupancy start later and
+    // still drain their counter).
+    const int min_blocks = (static_cast<int>(bk::kWorkCtrs) + P->tune.waves_per_block - 1) /
+                           P->tune.waves_per_block;
+    P->group_blocks.push_back(std::max(occ * n_cu, min_blocks));
     P->group_lds.push_back(lds);
   }
   if (!P->group_first.empty()) P->resolved_blocks_per_cu = min_occ;
@@ -1146,7 +1174,9 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   }
   const size_t btab_bytes = P->h_btab.size();
 
-  P->n_counter_words = 1 + static_cast<uint32_t>(P->group_first.size()) + n_fam;
+  P->work_word = (1 + n_fam + 31) / 32 * 32;
+  P->n_counter_words = P->work_word + static_cast<uint32_t>(P->group_first.size()) *
+                                         bk::kWorkCtrs * (bk::kCtrStride / 2);
   size_t off = 0;
   const size_t o_fam = off; off = align256(off + std::max<size_t>(n_fam, 1) * sizeof(Fam));
   const size_t o_g01 = off; off = align256(off + std::max<size_t>(n_pad, 1) * 8);
@@ -1315,8 +1345,9 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
     hipLaunchKernelGGL(bk::kernel_for(nf, P->tune.engine), dim3(P->group_blocks[gi]), dim3(P->threads),
                        P->group_lds[gi], P->stream, a, P->d_fam + f0, P->d_g01, P->d_gk, P->d_gaux, P->d_gid,
                        P->d_gnm, P->d_score, P->d_hits, P->d_counters,
-                       reinterpret_cast<unsigned int*>(P->d_counters + 1 + gi),
-                       P->d_counters + 1 + P->group_first.size() + f0,
+                       reinterpret_cast<unsigned int*>(P->d_counters + P->work_word) +
+                           gi * bk::kWorkCtrs * bk::kCtrStride,
+                       P->d_counters + 1 + f0,
                        reinterpret_cast<const int4*>(P->d_btab));
     BGO_HIP(hipGetLastError());
   }
@@ -1453,7 +1484,7 @@ extern "C" int bgo_plan_family_stats(const bgo_plan* P, int32_t cap, int32_t* n_
   if (n_fam) {
     BGO_HIP(hipSetDevice(P->device));
     BGO_HIP(hipStreamSynchronize(P->stream));
-    BGO_HIP(hipMemcpyAsync(w.data(), P->d_counters + 1 + P->group_first.size(),
+    BGO_HIP(hipMemcpyAsync(w.data(), P->d_counters + 1,
                            n_fam * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->stream));
     BGO_HIP(hipStreamSynchronize(P->stream));
   }
