@@ -37,6 +37,7 @@ class Tuning(C.Structure):
     def as_dict(self) -> dict:
         d = {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
         d["interleave"] = int(self.reserved[0])
+        d["direct_max"] = int(self.reserved[1])
         return d
 
 

@@ -87,6 +87,9 @@ def _tuning_struct(tuning: Optional[dict]) -> Optional[_lib.Tuning]:
         if k == "interleave":
             t.reserved[0] = int(v)
             continue
+        if k == "direct_max":
+            t.reserved[1] = int(v)
+            continue
         setattr(t, k, int(v))
     return t
 

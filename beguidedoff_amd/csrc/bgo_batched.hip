@@ -39,6 +39,7 @@ namespace bk {
 
 constexpr int kMaxFam = 8;     // PAM families per kernel launch
 constexpr int kUnroll = 8;     // guides per compare step
+constexpr int kDirectMax = 4;  // guides of a family compared in place (VALU engine)
 constexpr int kMaxBlock = 256; // threads per block (4 waves)
 constexpr uint32_t kWin = 0xFFFFFu;
 // Work items are handed out by kWorkCtrs counters per launch group, each in a
@@ -54,7 +55,9 @@ struct Fam {
   uint32_t plen;            // PAM length 1..6
   uint32_t g_begin, g_end;  // padded guide range
   uint32_t kmax;            // largest budget among the family's guides
-  uint32_t pad[3];
+  uint32_t g_live;          // end of the family's real (unpadded) guides
+  uint32_t direct;          // 1: windows compared in place, without the buffer
+  uint32_t pad;
 };
 
 struct Args {
@@ -204,6 +207,57 @@ __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, u
         if (ballot(cand)) record_candidates(a, t, s, gi, pos, W0, W1, ci, cand, lane);
       }
     }
+  }
+}
+
+// Compare this lane's passing windows (bits of pw in block bb) in place with
+// the NG (<= kDirectMax) guides of family F: the guides take one wide scalar
+// load (the family's table is padded to kUnroll), and each pass tests the
+// next window of every lane that has one against them, so the family needs
+// no window buffer, appends or padded compares.  Returns the number of
+// windows compared.
+template <int NG>
+__device__ inline uint32_t direct_passes(const Args& a, const Tab& t, const Fam& F, uint32_t s,
+                                         uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
+                                         uint32_t lane) {
+  const uint32_t kmax = F.kmax, g_begin = F.g_begin;
+  const uint4* __restrict__ gq = reinterpret_cast<const uint4*>(t.g01 + 2 * g_begin);
+  const uint4 q0 = gq[0], q1 = gq[1];
+  const uint32_t gx[4] = {q0.x, q0.z, q1.x, q1.z}, gy[4] = {q0.y, q0.w, q1.y, q1.w};
+  uint32_t n = 0;
+  for (uint64_t act = ballot(pw != 0); act; act = ballot(pw != 0)) {
+    n += static_cast<uint32_t>(__popcll(act));
+    const bool has = pw != 0;
+    const uint32_t j = has ? static_cast<uint32_t>(__builtin_ctz(pw)) : 0u;
+    const uint32_t W0 = funnel(qb.x, qa.x, j) & kWin, W1 = funnel(qb.y, qa.y, j) & kWin;
+    uint32_t m = __popc((W0 ^ gx[0]) | (W1 ^ gy[0]));
+#pragma unroll
+    for (int i = 1; i < NG; ++i)
+      m = min(m, static_cast<uint32_t>(__popc((W0 ^ gx[i]) | (W1 ^ gy[i]))));
+    if (ballot(has && m <= kmax)) {  // rare: per-guide budgets and the exact path
+      const uint32_t pos = (bb << 5) + j;
+#pragma nounroll
+      for (uint32_t i = 0; i < NG; ++i) {
+        const uint32_t g = g_begin + i;
+        const uint32_t ci = __popc((W0 ^ t.g01[2 * g]) | (W1 ^ t.g01[2 * g + 1]));
+        const bool cand = has && static_cast<int>(ci) <= t.gk[g];
+        if (ballot(cand)) record_candidates(a, t, s, g, pos, W0, W1, ci, cand, lane);
+      }
+    }
+    pw &= pw - 1;
+  }
+  return n;
+}
+
+__device__ inline uint32_t direct_family(const Args& a, const Tab& t, const Fam& F, uint32_t s,
+                                         uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
+                                         uint32_t lane) {
+  static_assert(kDirectMax == 4, "direct_family dispatches 1..4 guides");
+  switch (F.g_live - F.g_begin) {
+    case 1: return direct_passes<1>(a, t, F, s, pw, bb, qa, qb, lane);
+    case 2: return direct_passes<2>(a, t, F, s, pw, bb, qa, qb, lane);
+    case 3: return direct_passes<3>(a, t, F, s, pw, bb, qa, qb, lane);
+    default: return direct_passes<4>(a, t, F, s, pw, bb, qa, qb, lane);
   }
 }
 
@@ -661,7 +715,14 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
 #pragma nounroll
     for (uint32_t f = 0; f < NF; ++f) {
       const uint32_t pw = w.pws[f * 64 + lane];
-      if (ballot(pw != 0)) append_family<ENGINE>(a, t, w, f, s, pw, bb, qa, qb, cb, lane);
+      if (ballot(pw != 0)) {
+        if (ENGINE == 1 && t.fam[f].direct) {
+          const uint32_t n = direct_family(a, t, t.fam[f], s, pw, bb, qa, qb, lane);
+          if (lane == 0) w.wstats[f] += n;
+        } else {
+          append_family<ENGINE>(a, t, w, f, s, pw, bb, qa, qb, cb, lane);
+        }
+      }
     }
   }
 #pragma nounroll
@@ -800,6 +861,7 @@ struct bgo_plan {
   std::vector<int32_t> h_gk;
   bgo_tuning tune{};
   bool engine_auto = false;  // engine chosen per guide set
+  int direct_max = 4;        // VALU engine: families of <= this many guides compare in place
   // families, grouped into launches of <= kMaxFam
   std::vector<Fam> fams;              // all families (g ranges into the padded table)
   std::vector<int32_t> fam_guides;    // real guides per family
@@ -961,6 +1023,9 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   if (r.waves_per_block <= 0) r.waves_per_block = 4;
   if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
     return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");
+  // reserved[1]: 0 = default (4), -1 = never, n > 0 = at most n guides
+  // (at most kDirectMax: the direct compare holds a family's guides in SGPRs)
+  P->direct_max = r.reserved[1] == 0 ? bk::kDirectMax : std::clamp(r.reserved[1], 0, bk::kDirectMax);
   P->engine_auto = r.engine <= 0;
   if (r.engine <= 0) r.engine = 1;
   if (r.engine < 1 || r.engine > 3) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
@@ -1100,7 +1165,11 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       gnm.push_back(0);
     }
     F.g_end = static_cast<uint32_t>(gk.size());
+    F.g_live = F.g_begin + static_cast<uint32_t>(members[f].size());
     F.kmax = kmax;
+    // Families of at most direct_max guides skip the window buffer (VALU
+    // engine): in-place compares cost less than appends and a padded compare.
+    F.direct = P->tune.engine == 1 && static_cast<int>(members[f].size()) <= P->direct_max;
     P->fams.push_back(F);
     P->fam_guides.push_back(static_cast<int32_t>(members[f].size()));
   }
@@ -1470,6 +1539,7 @@ extern "C" int bgo_plan_get_tuning(const bgo_plan* P, bgo_tuning* out) {
   if (!P || !out) return fail(BGO_ERR_ARG, "null pointer");
   *out = P->tune;
   if (out->blocks_per_cu <= 0) out->blocks_per_cu = P->resolved_blocks_per_cu;
+  out->reserved[1] = P->direct_max > 0 ? P->direct_max : -1;
   return BGO_OK;
 }
 

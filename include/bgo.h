@@ -164,7 +164,11 @@ typedef struct bgo_tuning {
                                 guides, else VALU), 1 VALU xor/popcount,
                                 2 int8 MFMA, 3 fp4 MFMA */
   int32_t guide_unroll;      /* output: guides per compare step (8 VALU, 32 MFMA) */
-  int32_t reserved[3];       /* [0]: 1 = work items alternate strands */
+  int32_t reserved[3];       /* [0]: 1 = work items alternate strands;
+                                [1]: VALU engine, PAM families of at most this
+                                many guides compare their windows in place
+                                instead of through the window buffer
+                                (0 = default 4, -1 = never, at most 4) */
 } bgo_tuning;
 
 typedef struct bgo_plan bgo_plan;

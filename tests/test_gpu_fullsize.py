@@ -111,3 +111,24 @@ def test_8192_guides_engines_agree(mixed):
     sets = [hit_set(scan(dg, specs, tuning={"engine": e}).hits) for e in (1, 3)]
     assert sets[0] == sets[1]
     assert len(sets[0]) > 10_000
+
+
+def test_small_sets_direct_compare_equals_buffer(mixed, planted):
+    """Families of at most 4 guides compare their windows in place (VALU
+    engine); the sites equal those of the window-buffer path and of fp4, and
+    a direct scan of 4 planted guides finds every one of their plants."""
+    w, dg = mixed
+    by_pam = {}
+    for x in w.guides:
+        by_pam.setdefault(x.pam, []).append(x)
+    specs = specs_of([x for xs in by_pam.values() for x in xs[:2]], 5)
+    r = scan(dg, specs, tuning={"engine": 1})
+    assert r.tuning["direct_max"] == 4 and all(n == 2 for n, _ in r.families)
+    direct = hit_set(r.hits)
+    assert direct == hit_set(scan(dg, specs, tuning={"engine": 1, "direct_max": -1}).hits)
+    assert direct == hit_set(scan(dg, specs, tuning={"engine": 3}).hits)
+    assert len(direct) > 1000
+    w, dg = planted
+    sub = hit_set(scan(dg, specs_of(w.guides[:4], 4), tuning={"engine": 1}).hits)
+    full = hit_set(scan(dg, specs_of(w.guides, 4), tuning={"engine": 3}).hits)
+    assert sub == {h for h in full if h[0] < 4} and len(sub) >= 4 * 40

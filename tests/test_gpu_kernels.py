@@ -105,7 +105,13 @@ def test_plain_small_config(gpu):
     assert plain_set(w.genome, guides, 3) == oracle_set(w.genome, guides, 3)
 
 
-ENGINES = [1, 2, 3]   # 1: VALU xor/popcount, 2: int8 MFMA, 3: fp4 MFMA compare
+# Compare engines as tunings: 1 VALU xor/popcount (families of <= 4 guides
+# compare in place by default; direct_max -1 sends every family through the
+# window buffer, 1000 is capped at 4), 2 int8 MFMA, 3 fp4 MFMA.
+ENGINE_TUNINGS = {"valu": {"engine": 1}, "valu_buffered": {"engine": 1, "direct_max": -1},
+                  "valu_direct": {"engine": 1, "direct_max": 1000}, "i8": {"engine": 2},
+                  "fp4": {"engine": 3}}
+ENGINES = [pytest.param(t, id=k) for k, t in ENGINE_TUNINGS.items()]
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -116,7 +122,7 @@ def test_batched_matches_oracle_small(gpu, seed, engine):
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"]) if "N" in "".join(
         s for _, s in contigs) else DeviceGenome(genome)
     nm = genome if dg.has_n_mask else {k: v for k, v in genome.items() if not k.startswith("n_mask")}
-    r = scan(dg, [GuideSpec(s, p, max_mm) for s, p in guides], tuning={"engine": engine})
+    r = scan(dg, [GuideSpec(s, p, max_mm) for s, p in guides], tuning=dict(engine))
     assert hit_set(r.hits) == oracle_set(nm, guides, max_mm)
 
 
@@ -132,12 +138,12 @@ def test_batched_mixed_budgets_many_families_ranges(gpu, engine):
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
-    r = scan(dg, specs, tuning={"engine": engine})
+    r = scan(dg, specs, tuning=dict(engine))
     assert len(r.families) == 12
     assert hit_set(r.hits) == oracle_set(genome, guides, None, mm_per=budgets)
     L = genome["total_bp"]
     for (b, e), strands in [((1000, 17_000), 3), ((0, L // 2), 1), ((L // 3, L), 2)]:
-        r = scan(dg, specs, pos_begin=b, pos_end=e, strands=strands, tuning={"engine": engine})
+        r = scan(dg, specs, pos_begin=b, pos_end=e, strands=strands, tuning=dict(engine))
         want = oracle_set(genome, guides, None, mm_per=budgets, pos=(b, e),
                           strands=[s for s in (0, 1) if strands & (1 << s)])
         assert hit_set(r.hits) == want
@@ -151,7 +157,7 @@ def test_batched_capacity_rerun_and_fused_epilogue(gpu, engine):
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     specs = [GuideSpec(s, p, 6, ["ABE", "CBE"][i % 2], (int(rng.integers(1, 6)), int(rng.integers(6, 21))))
              for i, (s, p) in enumerate(guides)]
-    r = scan(dg, specs, hit_capacity=3, tuning={"engine": engine})
+    r = scan(dg, specs, hit_capacity=3, tuning=dict(engine))
     assert r.launches == 2 and r.n_hits == r.hits.size > 3
     assert hit_set(r.hits) == oracle_set(genome, guides, 6)
     h = r.hits
@@ -181,7 +187,7 @@ def test_batched_family_window_counts(gpu, engine):
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     pams = ["NGG", "NNNRRT", "NG", "TTTV"]
     specs = [GuideSpec(guides[0][0], p, 3) for p in pams]
-    r = scan(dg, specs, tuning={"engine": engine})
+    r = scan(dg, specs, tuning=dict(engine))
     L = genome["total_bp"]
     from beguidedoff_amd.pam import IUPAC_MASK
     for (n_g, windows), pam in zip(r.families, pams):
@@ -201,7 +207,7 @@ def test_graph_replays_match_eager(gpu, engine):
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     specs = [GuideSpec(s, p, max_mm) for s, p in guides]
-    plan = Plan(dg, specs, hit_capacity=1 << 16, tuning={"engine": engine})
+    plan = Plan(dg, specs, hit_capacity=1 << 16, tuning=dict(engine))
     plan.launch()
     n, _ = plan.wait()
     eager = hit_set(plan.fetch(n))
@@ -253,8 +259,8 @@ def test_mfma_engine_many_guides_high_budget(gpu):
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
     want = oracle_set(genome, guides, None, mm_per=budgets)
-    for engine in ENGINES:
-        assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
+    for engine in ENGINE_TUNINGS.values():
+        assert hit_set(scan(dg, specs, tuning=dict(engine)).hits) == want
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -278,7 +284,7 @@ def test_budget_and_composition_extremes(gpu, engine):
     specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
     want = oracle_set(genome, guides, None, mm_per=budgets)
     assert len(want) > 1000
-    assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
+    assert hit_set(scan(dg, specs, tuning=dict(engine)).hits) == want
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -303,7 +309,7 @@ def test_odd_guide_tile_counts_with_planted_sites(gpu, engine):
     specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
     want = oracle_set(genome, guides, None, mm_per=budgets)
     assert len({g for g, *_ in want}) >= 40
-    assert hit_set(scan(dg, specs, tuning={"engine": engine}).hits) == want
+    assert hit_set(scan(dg, specs, tuning=dict(engine)).hits) == want
 
 
 def test_fetch_view_equals_fetch(gpu):
