@@ -39,7 +39,8 @@ namespace bk {
 
 constexpr int kMaxFam = 8;     // PAM families per kernel launch
 constexpr int kUnroll = 8;     // guides per compare step
-constexpr int kDirectMax = 4;  // guides of a family compared in place (VALU engine)
+constexpr int kDirectMax = 8;  // guides of a family compared in place (VALU engine)
+constexpr int kDirectSparse = 4;  // default limit for PAMs passing < 5% of positions
 constexpr int kMaxBlock = 256; // threads per block (4 waves)
 constexpr uint32_t kWin = 0xFFFFFu;
 // Work items are handed out by kWorkCtrs counters per launch group, each in a
@@ -211,11 +212,12 @@ __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, u
 }
 
 // Compare this lane's passing windows (bits of pw in block bb) in place with
-// the NG (<= kDirectMax) guides of family F: the guides take one wide scalar
-// load (the family's table is padded to kUnroll), and each pass tests the
-// next window of every lane that has one against them, so the family needs
-// no window buffer, appends or padded compares.  Returns the number of
-// windows compared.
+// the NG (<= kDirectMax = kUnroll) guides of family F: the guides take wide
+// scalar loads (the family's table is padded to kUnroll; guides 5..8 move to
+// VGPRs, which the VALU engine has to spare, rather than add SGPRs), and each
+// pass tests the next window of every lane that has one against them, so
+// the family needs no window buffer, appends or padded compares.  Returns
+// the number of windows compared.
 template <int NG>
 __device__ inline uint32_t direct_passes(const Args& a, const Tab& t, const Fam& F, uint32_t s,
                                          uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
@@ -223,7 +225,15 @@ __device__ inline uint32_t direct_passes(const Args& a, const Tab& t, const Fam&
   const uint32_t kmax = F.kmax, g_begin = F.g_begin;
   const uint4* __restrict__ gq = reinterpret_cast<const uint4*>(t.g01 + 2 * g_begin);
   const uint4 q0 = gq[0], q1 = gq[1];
-  const uint32_t gx[4] = {q0.x, q0.z, q1.x, q1.z}, gy[4] = {q0.y, q0.w, q1.y, q1.w};
+  uint4 q2 = make_uint4(0, 0, 0, 0), q3 = q2;
+  if constexpr (NG > 4) {
+    q2 = gq[2];
+    q3 = gq[3];
+    asm volatile("" : "+v"(q2.x), "+v"(q2.y), "+v"(q2.z), "+v"(q2.w));
+    asm volatile("" : "+v"(q3.x), "+v"(q3.y), "+v"(q3.z), "+v"(q3.w));
+  }
+  const uint32_t gx[8] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
+  const uint32_t gy[8] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w};
   uint32_t n = 0;
   for (uint64_t act = ballot(pw != 0); act; act = ballot(pw != 0)) {
     n += static_cast<uint32_t>(__popcll(act));
@@ -252,12 +262,16 @@ __device__ inline uint32_t direct_passes(const Args& a, const Tab& t, const Fam&
 __device__ inline uint32_t direct_family(const Args& a, const Tab& t, const Fam& F, uint32_t s,
                                          uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
                                          uint32_t lane) {
-  static_assert(kDirectMax == 4, "direct_family dispatches 1..4 guides");
+  static_assert(kDirectMax == 8 && kUnroll == 8, "direct_family dispatches 1..8 guides");
   switch (F.g_live - F.g_begin) {
     case 1: return direct_passes<1>(a, t, F, s, pw, bb, qa, qb, lane);
     case 2: return direct_passes<2>(a, t, F, s, pw, bb, qa, qb, lane);
     case 3: return direct_passes<3>(a, t, F, s, pw, bb, qa, qb, lane);
-    default: return direct_passes<4>(a, t, F, s, pw, bb, qa, qb, lane);
+    case 4: return direct_passes<4>(a, t, F, s, pw, bb, qa, qb, lane);
+    case 5: return direct_passes<5>(a, t, F, s, pw, bb, qa, qb, lane);
+    case 6: return direct_passes<6>(a, t, F, s, pw, bb, qa, qb, lane);
+    case 7: return direct_passes<7>(a, t, F, s, pw, bb, qa, qb, lane);
+    default: return direct_passes<8>(a, t, F, s, pw, bb, qa, qb, lane);
   }
 }
 
@@ -861,7 +875,9 @@ struct bgo_plan {
   std::vector<int32_t> h_gk;
   bgo_tuning tune{};
   bool engine_auto = false;  // engine chosen per guide set
-  int direct_max = 4;        // VALU engine: families of <= this many guides compare in place
+  int direct_max = 0;        // VALU engine: families of <= this many guides compare in
+                             // place (0: 8 for PAMs passing >= 5% of positions, else 4)
+  bool direct_off = false;   // no family compares in place
   // families, grouped into launches of <= kMaxFam
   std::vector<Fam> fams;              // all families (g ranges into the padded table)
   std::vector<int32_t> fam_guides;    // real guides per family
@@ -1023,9 +1039,10 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   if (r.waves_per_block <= 0) r.waves_per_block = 4;
   if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
     return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");
-  // reserved[1]: 0 = default (4), -1 = never, n > 0 = at most n guides
-  // (at most kDirectMax: the direct compare holds a family's guides in SGPRs)
-  P->direct_max = r.reserved[1] == 0 ? bk::kDirectMax : std::clamp(r.reserved[1], 0, bk::kDirectMax);
+  // reserved[1]: 0 = default, -1 = never, n > 0 = at most n guides
+  // 0 = by PAM density (below), n > 0 = n for every family (at most kDirectMax)
+  P->direct_max = r.reserved[1] == 0 ? 0 : std::clamp(r.reserved[1], 0, bk::kDirectMax);
+  P->direct_off = r.reserved[1] < 0;
   P->engine_auto = r.engine <= 0;
   if (r.engine <= 0) r.engine = 1;
   if (r.engine < 1 || r.engine > 3) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
@@ -1167,9 +1184,25 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     F.g_end = static_cast<uint32_t>(gk.size());
     F.g_live = F.g_begin + static_cast<uint32_t>(members[f].size());
     F.kmax = kmax;
-    // Families of at most direct_max guides skip the window buffer (VALU
-    // engine): in-place compares cost less than appends and a padded compare.
-    F.direct = P->tune.engine == 1 && static_cast<int>(members[f].size()) <= P->direct_max;
+    // Families of few guides skip the window buffer (VALU engine): in-place
+    // compares cost less than appends and a padded compare.  By default the
+    // limit is 8 guides for PAMs passing >= 5% of positions of a 41% GC
+    // genome (NG 20%, NNNRRT 7.4%: appends are their main cost) and 4 for
+    // sparser ones (NGG 4.2%, NNGRRT 1.5%), the crossovers measured on a
+    // 3.09 Gbp genome of that composition.
+    static constexpr double kBaseFreq[4] = {0.295, 0.205, 0.205, 0.295};  // A C G T
+    double pass = 1.0;
+    for (int i = 0; i < 6; ++i) {
+      const uint32_t nib = (F.nibs >> (4 * i)) & 0xFu;
+      double p = 0.0;
+      for (int b = 0; b < 4; ++b) p += (nib >> b) & 1u ? kBaseFreq[b] : 0.0;
+      pass *= p;
+    }
+    const int limit = P->direct_off ? 0
+                      : P->direct_max ? P->direct_max
+                      : pass >= 0.05  ? bk::kDirectMax
+                                      : bk::kDirectSparse;
+    F.direct = P->tune.engine == 1 && static_cast<int>(members[f].size()) <= limit;
     P->fams.push_back(F);
     P->fam_guides.push_back(static_cast<int32_t>(members[f].size()));
   }
@@ -1539,7 +1572,7 @@ extern "C" int bgo_plan_get_tuning(const bgo_plan* P, bgo_tuning* out) {
   if (!P || !out) return fail(BGO_ERR_ARG, "null pointer");
   *out = P->tune;
   if (out->blocks_per_cu <= 0) out->blocks_per_cu = P->resolved_blocks_per_cu;
-  out->reserved[1] = P->direct_max > 0 ? P->direct_max : -1;
+  out->reserved[1] = P->direct_off ? -1 : P->direct_max;
   return BGO_OK;
 }
 

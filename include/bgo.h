@@ -167,8 +167,10 @@ typedef struct bgo_tuning {
   int32_t reserved[3];       /* [0]: 1 = work items alternate strands;
                                 [1]: VALU engine, PAM families of at most this
                                 many guides compare their windows in place
-                                instead of through the window buffer
-                                (0 = default 4, -1 = never, at most 4) */
+                                instead of through the window buffer (1..8;
+                                -1 = never; 0 = 8 for PAMs that pass >= 5% of
+                                the positions of a 41% GC genome, 4 for
+                                sparser ones) */
 } bgo_tuning;
 
 typedef struct bgo_plan bgo_plan;

@@ -114,16 +114,18 @@ def test_8192_guides_engines_agree(mixed):
 
 
 def test_small_sets_direct_compare_equals_buffer(mixed, planted):
-    """Families of at most 4 guides compare their windows in place (VALU
-    engine); the sites equal those of the window-buffer path and of fp4, and
-    a direct scan of 4 planted guides finds every one of their plants."""
+    """Small PAM families compare their windows in place (VALU engine): with
+    6 guides per family the dense PAMs (NG, NNNRRT) do and the sparse ones
+    (NGG, NNGRRT) use the window buffer.  The sites equal those of the
+    all-buffer path and of fp4, and a direct scan of 4 planted guides finds
+    every one of their plants."""
     w, dg = mixed
     by_pam = {}
     for x in w.guides:
         by_pam.setdefault(x.pam, []).append(x)
-    specs = specs_of([x for xs in by_pam.values() for x in xs[:2]], 5)
+    specs = specs_of([x for xs in by_pam.values() for x in xs[:6]], 5)
     r = scan(dg, specs, tuning={"engine": 1})
-    assert r.tuning["direct_max"] == 4 and all(n == 2 for n, _ in r.families)
+    assert r.tuning["direct_max"] == 0 and all(n == 6 for n, _ in r.families)
     direct = hit_set(r.hits)
     assert direct == hit_set(scan(dg, specs, tuning={"engine": 1, "direct_max": -1}).hits)
     assert direct == hit_set(scan(dg, specs, tuning={"engine": 3}).hits)

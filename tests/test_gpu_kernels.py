@@ -105,9 +105,10 @@ def test_plain_small_config(gpu):
     assert plain_set(w.genome, guides, 3) == oracle_set(w.genome, guides, 3)
 
 
-# Compare engines as tunings: 1 VALU xor/popcount (families of <= 4 guides
-# compare in place by default; direct_max -1 sends every family through the
-# window buffer, 1000 is capped at 4), 2 int8 MFMA, 3 fp4 MFMA.
+# Compare engines as tunings: 1 VALU xor/popcount (by default families of up
+# to 8 guides of dense PAMs and 4 of sparse ones compare in place; direct_max
+# -1 sends every family through the window buffer, 1000 is capped at 8),
+# 2 int8 MFMA, 3 fp4 MFMA.
 ENGINE_TUNINGS = {"valu": {"engine": 1}, "valu_buffered": {"engine": 1, "direct_max": -1},
                   "valu_direct": {"engine": 1, "direct_max": 1000}, "i8": {"engine": 2},
                   "fp4": {"engine": 3}}
@@ -327,3 +328,31 @@ def test_fetch_view_equals_fetch(gpu):
         v = p.fetch_view(cap)
         assert np.array_equal(v, a[: v.size]) and v.size == (7 if cap else n)
     p.close()
+
+
+def test_direct_compare_every_guide_count(gpu):
+    """PAM families of 1..8 guides, dense and sparse PAMs, per-guide budgets and
+    planted near-copies: the in-place compare (every guide count), the window
+    buffer and fp4 report the oracle's sites."""
+    from brute import IUPAC, plant
+    rng = np.random.default_rng(41)
+    contigs, _, _ = build_case(41, length=(40_000, 60_000), n_guides=1)
+    pams = ["NG", "NNNRRT", "NRG", "NNG", "TG", "NNNNRT", "YG", "NGG"]
+    guides = []
+    for n, pam in enumerate(pams, start=1):
+        guides += [("".join(rng.choice(list("ACGT"), size=20)), pam) for _ in range(n)]
+    for gi in rng.choice(len(guides), size=30, replace=False):
+        q, pam = guides[int(gi)]
+        pam_seq = "".join(rng.choice(list(IUPAC[c])) for c in pam)
+        plant(rng, contigs, q, pam_seq, int(rng.integers(0, 5)), "+" if rng.random() < 0.5 else "-")
+    budgets = [int(rng.integers(0, 9)) for _ in guides]
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
+    want = oracle_set(genome, guides, None, mm_per=budgets)
+    assert len(want) > 100 and len({g for g, *_ in want}) >= 25
+    for t in ({"engine": 1}, {"engine": 1, "direct_max": 8}, {"engine": 1, "direct_max": -1},
+              {"engine": 1, "direct_max": 3}, {"engine": 3}):
+        r = scan(dg, specs, tuning=t)
+        assert sorted(n for n, _ in r.families) == list(range(1, 9)), t
+        assert hit_set(r.hits) == want, t
