@@ -1015,6 +1015,31 @@ Fam make_fam(const bgo_guide_desc& d) {
   return F;
 }
 
+// Fraction of the positions of a 41% GC genome whose bases pass the PAM.
+double pam_pass_rate(uint32_t nibs) {
+  static constexpr double kBaseFreq[4] = {0.295, 0.205, 0.205, 0.295};  // A C G T
+  double pass = 1.0;
+  for (int i = 0; i < 6; ++i) {
+    const uint32_t nib = (nibs >> (4 * i)) & 0xFu;
+    double p = 0.0;
+    for (int b = 0; b < 4; ++b) p += (nib >> b) & 1u ? kBaseFreq[b] : 0.0;
+    pass *= p;
+  }
+  return pass;
+}
+
+// Guides up to which a family compares its windows in place (VALU engine):
+// in-place compares cost less than appends and a padded compare.  By default
+// 8 for PAMs passing >= 5% of the positions of a 41% GC genome (NG 20%,
+// NNNRRT 7.4%: appends are their main cost) and 4 for sparser ones (NGG
+// 4.2%, NNGRRT 1.5%), the crossovers measured on a 3.09 Gbp genome of that
+// composition.
+int direct_limit(const bgo_plan* P, double pass) {
+  if (P->direct_off) return 0;
+  if (P->direct_max) return P->direct_max;
+  return pass >= 0.05 ? bk::kDirectMax : bk::kDirectSparse;
+}
+
 // Four fp4 (e2m1) codes summing to x, |x| <= 22 (magnitudes 6, 4, 3, 2, 1).
 bool fp4_sum4(int x, uint8_t out[4]) {
   const uint8_t sign = x < 0 ? 0x8 : 0x0;
@@ -1130,9 +1155,19 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   }
   const int n_fam = static_cast<int>(members.size());
   if (P->engine_auto) {
-    // fp4 matrix cores once families average 16 guides (a 32-guide tile is
-    // then at least half used); the VALU compare for sparser sets.
-    P->tune.engine = n_guides >= 16 * n_fam ? 3 : 1;
+    // The engine with the lower modelled cost.  Per family, in ms per unit of
+    // PAM pass rate on a 3.09 Gbp genome (fitted to single-family and mixed
+    // launches on MI355X): VALU in place 0.86 + 0.94 G, VALU through the
+    // window buffer 6.7 + 3.7 ceil(G / 8), fp4 8.1 + 4.0 ceil(G / 32).  The
+    // streaming both engines share drops out.
+    double valu = 0.0, fp4 = 0.0;
+    for (int f = 0; f < n_fam; ++f) {
+      const double pass = pam_pass_rate(proto[f].nibs);
+      const int g = static_cast<int>(members[f].size());
+      valu += pass * (g <= direct_limit(P, pass) ? 0.86 + 0.94 * g : 6.7 + 3.7 * ((g + 7) / 8));
+      fp4 += pass * (8.1 + 4.0 * ((g + 31) / 32));
+    }
+    P->tune.engine = fp4 < valu ? 3 : 1;
     P->tune.guide_unroll = P->tune.engine >= 2 ? 32 : bk::kUnroll;
   }
   P->fams.clear();
@@ -1184,25 +1219,8 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     F.g_end = static_cast<uint32_t>(gk.size());
     F.g_live = F.g_begin + static_cast<uint32_t>(members[f].size());
     F.kmax = kmax;
-    // Families of few guides skip the window buffer (VALU engine): in-place
-    // compares cost less than appends and a padded compare.  By default the
-    // limit is 8 guides for PAMs passing >= 5% of positions of a 41% GC
-    // genome (NG 20%, NNNRRT 7.4%: appends are their main cost) and 4 for
-    // sparser ones (NGG 4.2%, NNGRRT 1.5%), the crossovers measured on a
-    // 3.09 Gbp genome of that composition.
-    static constexpr double kBaseFreq[4] = {0.295, 0.205, 0.205, 0.295};  // A C G T
-    double pass = 1.0;
-    for (int i = 0; i < 6; ++i) {
-      const uint32_t nib = (F.nibs >> (4 * i)) & 0xFu;
-      double p = 0.0;
-      for (int b = 0; b < 4; ++b) p += (nib >> b) & 1u ? kBaseFreq[b] : 0.0;
-      pass *= p;
-    }
-    const int limit = P->direct_off ? 0
-                      : P->direct_max ? P->direct_max
-                      : pass >= 0.05  ? bk::kDirectMax
-                                      : bk::kDirectSparse;
-    F.direct = P->tune.engine == 1 && static_cast<int>(members[f].size()) <= limit;
+    F.direct = P->tune.engine == 1 &&
+               static_cast<int>(members[f].size()) <= direct_limit(P, pam_pass_rate(F.nibs));
     P->fams.push_back(F);
     P->fam_guides.push_back(static_cast<int32_t>(members[f].size()));
   }

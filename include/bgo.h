@@ -160,8 +160,9 @@ typedef struct bgo_tuning {
   int32_t chunk_positions;   /* positions per work item (multiple of 64) */
   int32_t waves_per_block;   /* 1, 2 or 4 */
   int32_t blocks_per_cu;     /* resident blocks requested per CU */
-  int32_t engine;            /* 0 auto (fp4 MFMA when PAM families average >= 16
-                                guides, else VALU), 1 VALU xor/popcount,
+  int32_t engine;            /* 0 auto (VALU or fp4 MFMA, whichever a per-family
+                                cost model of the guide counts and PAM pass
+                                rates favours), 1 VALU xor/popcount,
                                 2 int8 MFMA, 3 fp4 MFMA */
   int32_t guide_unroll;      /* output: guides per compare step (8 VALU, 32 MFMA) */
   int32_t reserved[3];       /* [0]: 1 = work items alternate strands;

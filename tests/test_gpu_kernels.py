@@ -356,3 +356,26 @@ def test_direct_compare_every_guide_count(gpu):
         r = scan(dg, specs, tuning=t)
         assert sorted(n for n, _ in r.families) == list(range(1, 9)), t
         assert hit_set(r.hits) == want, t
+
+
+def test_auto_engine_choice(gpu):
+    """Engine 0 takes the cheaper engine by the per-family cost model: the VALU
+    compare for small sets (in-place families) and fp4 for large families."""
+    contigs, _, _ = build_case(3, length=(5_000, 8_000), n_guides=1)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    rng = np.random.default_rng(2)
+
+    def engine_for(counts):
+        specs = [GuideSpec("".join(rng.choice(list("ACGT"), size=20)), pam, 4)
+                 for pam, n in counts.items() for _ in range(n)]
+        p = Plan(dg, specs, hit_capacity=1 << 12)
+        try:
+            return p.tuning()["engine"]
+        finally:
+            p.close()
+
+    assert engine_for({"NGG": 1, "NNNRRT": 1, "NNGRRT": 1, "NG": 1}) == 1
+    assert engine_for({"NGG": 36, "NNNRRT": 21, "NNGRRT": 16, "NG": 7}) == 1
+    assert engine_for({"NGG": 64, "NNNRRT": 64, "NNGRRT": 64, "NG": 64}) == 3
+    assert engine_for({"NGG": 512, "NNNRRT": 256, "NNGRRT": 128, "NG": 128}) == 3
