@@ -14,11 +14,12 @@ Launches alternate between `pipeline` plans with their own HIP streams, so a
 segment can be enqueued while the previous one still runs and the host-side
 dispatch of finished segments overlaps device work.
 
-Segment count: each launch of the persistent kernel pays a ramp-up and a tail
-(0.2-0.35 ms on a 3.09 Gbp genome), so more segments shorten the wait to join
-but lengthen every request's pass.  On that genome with 1-500-guide requests at
-2-32 req/s, 4 segments gave p50 6.1-7.2 ms against 8.1-8.7 ms for 16, at the
-same p99.
+Segment count: more segments shorten the wait to join but add a launch's
+start-up and tail to every request's pass (about 0.015 ms to start a
+persistent grid, plus the last work items).  On a 3.09 Gbp genome with
+1-500-guide requests at 2 and 32 req/s, 8 segments gave mean latencies of
+6.2-6.6 ms and p99 of 17.7-19.9 ms, against 6.5-7.2 and 19.1-21.4 ms for 4
+and 6.8-7.6 and 19.3-24.4 ms for 2.
 
 Admission: each request reserves an estimated number of hit records (from
 its PAMs and mismatch budgets); requests are admitted in arrival order while
@@ -87,7 +88,7 @@ class Request:
 
 
 class ScanServer:
-    def __init__(self, genome: DeviceGenome, n_segments: int = 4, hit_budget: int = 1 << 24,
+    def __init__(self, genome: DeviceGenome, n_segments: int = 8, hit_budget: int = 1 << 24,
                  pipeline: int = 2, tuning: Optional[dict] = None, min_reserve: int = 1024):
         if n_segments < 1 or pipeline < 1:
             raise ValueError("n_segments and pipeline must be >= 1")

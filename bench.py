@@ -75,7 +75,7 @@ def mfma_ops(families) -> int:
 # counters' reading for a copy kernel of known size (profiles/
 # batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md).  Keyed by (workload,
 # engine); None when not profiled.
-TRAFFIC_BYTES = {("hg38_mixed", 3): 1_606_173_344}
+TRAFFIC_BYTES = {("hg38_mixed", 3): 1_606_135_584}
 
 
 def copy_probe(mbytes: int) -> None:
@@ -339,7 +339,7 @@ def serving_leg(dg, pool, max_mm: int, rates, duration: float, seed: int = 0) ->
     gbp = dg.total_bp / 1e9
     out = {}
     for rate in rates:
-        with ScanServer(dg, n_segments=4, hit_budget=1 << 24, pipeline=2) as srv:
+        with ScanServer(dg, n_segments=8, hit_budget=1 << 24, pipeline=2) as srv:
             t0 = time.perf_counter()
             handles = []
             t_next = t0
@@ -389,6 +389,9 @@ def main() -> int:
     ap.add_argument("--serving-duration", type=float, default=15.0,
                     help="seconds of Poisson arrivals per rate")
     ap.add_argument("--no-engines", action="store_true", help="skip the VALU/MFMA comparison")
+    ap.add_argument("--guides", type=int, default=0,
+                    help="scan a fixed random subset of this many of the workload's guides "
+                         "(small request sizes; 0 = all); skips the CPU legs")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration; "
                          "a process of its own: torch carries its own HIP runtime)")
@@ -432,6 +435,11 @@ def main() -> int:
 
     specs = [GuideSpec(x.guide_seq, x.pam, args.max_mm, x.editor_type,
                        (x.edit_window_start, x.edit_window_end)) for x in w.guides]
+    if args.guides > 0:
+        keep = np.sort(np.random.default_rng(1).permutation(len(specs))[:args.guides])
+        specs = [specs[i] for i in keep]
+        args.no_cpu = True  # the CPU legs time the whole workload's guide set
+        log(f"guide subset: {len(specs)} of {len(w.guides)} guides")
     t0 = time.perf_counter()
     dg = DeviceGenome(g, g["n_mask_le_fwd"], g["n_mask_le_rc"])
     log(f"genome resident in HBM ({dg.device_bytes() / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s")
