@@ -57,23 +57,31 @@ def guide_descs(specs: Sequence[GuideSpec]) -> np.ndarray:
         codes = encode_dna_rows([seqs[i] for i in idx], L)[:, :k].astype(np.uint64)
         packed[idx] = (codes << (2 * np.arange(k, dtype=np.uint64))).sum(axis=1, dtype=np.uint64)
     d["seq2bit"] = packed
-    mm = np.array([int(s.max_mm) for s in specs], dtype=np.int64)
+    mm = np.fromiter((s.max_mm for s in specs), dtype=np.int64, count=n)
     bad = np.flatnonzero((mm < 0) | (mm > 20))
     if bad.size:
         raise ValueError(f"max_mm must be in 0..20, got {specs[int(bad[0])].max_mm}")
     d["max_mm"] = mm
-    pam_of = {p: encode_pam_mask(p) for p in {s.pam for s in specs}}
-    d["pam_mask"] = np.stack([pam_of[s.pam] for s in specs])
-    d["pam_len"] = [len(s.pam) for s in specs]
-    editor_of = {}
-    for e in {s.editor_type for s in specs}:
+    # PAMs, editors and windows: a table per distinct value, indexed per guide
+    pam_ix: Dict[str, int] = {}
+    pi = np.fromiter((pam_ix.setdefault(s.pam, len(pam_ix)) for s in specs), dtype=np.int64,
+                     count=n)
+    d["pam_mask"] = np.stack([encode_pam_mask(p) for p in pam_ix])[pi]
+    d["pam_len"] = np.array([len(p) for p in pam_ix], dtype=np.int64)[pi]
+    ed_ix: Dict[str, int] = {}
+    ei = np.fromiter((ed_ix.setdefault(s.editor_type, len(ed_ix)) for s in specs),
+                     dtype=np.int64, count=n)
+    codes = []
+    for e in ed_ix:
         target = EDITABLE_BASE.get(e)
         if target is None:
             raise ValueError(f"Unknown editor_type: {e}")
-        editor_of[e] = _EDITOR_CODE[target]
-    d["editor"] = [editor_of[s.editor_type] for s in specs]
-    window_of = {w: edit_window_mask(*w) for w in {tuple(s.window) for s in specs}}
-    d["edit_mask"] = [window_of[tuple(s.window)] for s in specs]
+        codes.append(_EDITOR_CODE[target])
+    d["editor"] = np.array(codes, dtype=np.int64)[ei]
+    win_ix: Dict[tuple, int] = {}
+    wi = np.fromiter((win_ix.setdefault(tuple(s.window), len(win_ix)) for s in specs),
+                     dtype=np.int64, count=n)
+    d["edit_mask"] = np.array([edit_window_mask(*w) for w in win_ix], dtype=np.int64)[wi]
     return d
 
 

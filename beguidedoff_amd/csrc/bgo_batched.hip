@@ -1308,7 +1308,10 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   const size_t o_cnt = off; off = align256(off + P->n_counter_words * 8);
   const size_t o_btab = off; off = align256(off + std::max<size_t>(btab_bytes, 16));
   if (off > P->mem_bytes) {
-    const size_t want = std::max(off, P->mem_bytes * 3 / 2);
+    // At least 4 MB (tables of ~16K padded guides): freeing device memory waits
+    // for the device's running kernels, which would stall a serving loop
+    // that swaps guide sets while another plan's scan runs.
+    const size_t want = std::max({off, P->mem_bytes * 3 / 2, size_t{4} << 20});
     if (P->d_mem) (void)hipFree(P->d_mem);
     P->d_mem = nullptr;
     P->mem_bytes = 0;
@@ -1568,7 +1571,9 @@ extern "C" int bgo_plan_fetch_pinned(bgo_plan* P, int64_t cap, const bgo_hit** o
   if (cap >= 0) m = std::min(m, cap);
   m = std::max<int64_t>(m, 0);
   if (m > P->pinned_cap) {
-    const int64_t want = std::max<int64_t>(m, P->pinned_cap * 3 / 2);
+    // grows at least 2x from 64K records: freeing page-locked memory can wait
+    // for the device's running kernels
+    const int64_t want = std::max<int64_t>({m, P->pinned_cap * 2, int64_t{1} << 16});
     if (P->h_pinned) (void)hipHostFree(P->h_pinned);
     P->h_pinned = nullptr;
     P->pinned_cap = 0;

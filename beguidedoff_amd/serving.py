@@ -56,7 +56,15 @@ def _p_le_k(k: int, n: int = 20, p_mm: float = 0.75) -> float:
 
 def estimate_hits(specs: Sequence[GuideSpec], total_bp: int) -> float:
     """Expected sites of a guide set on a uniform random genome (both strands)."""
-    return sum(2.0 * total_bp * _pam_pass_rate(s.pam) * _p_le_k(s.max_mm) for s in specs)
+    per: Dict[tuple, float] = {}
+    total = 0.0
+    for s in specs:
+        k = (s.pam, s.max_mm)
+        v = per.get(k)
+        if v is None:
+            v = per[k] = 2.0 * total_bp * _pam_pass_rate(s.pam) * _p_le_k(s.max_mm)
+        total += v
+    return total
 
 
 @dataclass
@@ -99,7 +107,10 @@ class ScanServer:
         end = max(0, genome.total_bp - 26)
         self.bounds = [(end * k // n_segments, end * (k + 1) // n_segments)
                        for k in range(n_segments)]
-        self._plans = [Plan(genome, [], tuning=tuning, hit_capacity=1 << 16)
+        # 8 MB of hit records per plan to start; buffers grow at least 2x, since
+        # each reallocation frees device memory, which waits for the other
+        # plans' running segments
+        self._plans = [Plan(genome, [], tuning=tuning, hit_capacity=1 << 18)
                        for _ in range(pipeline)]
         self._plan_set = {id(p): () for p in self._plans}  # request ids a plan's tables hold
         self._free = collections.deque(self._plans)
@@ -194,7 +205,7 @@ class ScanServer:
         self._seg = (self._seg + 1) % self.n_segments
         cap = max(1 << 12, int(2 * sum(r.reserve for r in reqs) / self.n_segments))
         if cap > plan.hit_capacity:
-            plan.set_hit_capacity(cap)
+            plan.set_hit_capacity(max(cap, 2 * plan.hit_capacity))
         key = tuple(r.rid for r in reqs)
         if self._plan_set[id(plan)] != key:  # the active set changed: new guide tables
             plan.set_guides(specs, np.concatenate([r.descs for r in reqs]))
@@ -215,16 +226,23 @@ class ScanServer:
             self.reruns += 1
         self.busy_s += ms / 1e3
         hits = plan.fetch_view(n)
-        order = np.argsort(hits["guide"], kind="stable")
-        hits = hits[order]  # a copy: the plan's buffer may be refilled from here on
+        if len(mapping) == 1:  # one request: its records as they are
+            hits = hits.copy()  # the plan's buffer may be refilled from here on
+            g = None
+        else:
+            order = np.argsort(hits["guide"], kind="stable")
+            hits = hits[order]  # a copy, as above
+            g = hits["guide"]
         self._free.append(plan)
-        g = hits["guide"]
         now = time.perf_counter()
         finished = []
         for req, off, cnt in mapping:
-            lo, hi = np.searchsorted(g, [off, off + cnt])
-            if hi > lo:
-                part = hits[lo:hi].copy()
+            if g is None:
+                part = hits
+            else:
+                lo, hi = np.searchsorted(g, [off, off + cnt])
+                part = hits[lo:hi].copy() if hi > lo else hits[:0]
+            if part.size:
                 part["guide"] -= off
                 req.parts.append(part)
             req.retired += 1

@@ -126,3 +126,25 @@ def test_genome_converter(tmp_path):
     fa2.write_text(">chr1\n" + "ACGTNNACGT" * 5 + "\n")
     convert(str(fa2), str(tmp_path / "v.npz"), verbose=False)
     assert verify(str(tmp_path / "v.npz"), str(fa2), sample_n=100, verbose=False)
+
+
+def test_guide_descs_fields():
+    """Batched-scan guide descriptors: the first 20 bases 2 bits each (A0 C1 G2
+    T3, base i at bits 2i), IUPAC PAM nibbles and length, budget, editor target
+    (A for ABE, C for CBE/CGE) and the edit window's bit mask."""
+    from beguidedoff_amd.batched import GuideSpec, guide_descs
+    d = guide_descs([GuideSpec("ACGT" * 5, "NGG", 3, "ABE", (4, 8)),
+                     GuideSpec("T" * 21, "NNGRRT", 5, "CBE", (3, 9)),
+                     GuideSpec("ACGT" * 5, "NGG", 0, "CGE", (4, 8))])
+    assert d["seq2bit"].tolist() == [0xE4E4E4E4E4, (1 << 40) - 1, 0xE4E4E4E4E4]
+    assert d["pam_mask"].tolist() == [[15, 4, 4, 0, 0, 0], [15, 15, 4, 5, 5, 8],
+                                      [15, 4, 4, 0, 0, 0]]
+    assert d["pam_len"].tolist() == [3, 6, 3]
+    assert d["max_mm"].tolist() == [3, 5, 0]
+    assert d["editor"].tolist() == [0, 1, 1]
+    assert d["edit_mask"].tolist() == [0b11111000, 0b111111100, 0b11111000]
+    assert guide_descs([]).size == 0
+    with pytest.raises(ValueError, match="max_mm"):
+        guide_descs([GuideSpec("A" * 20, "NGG", 21)])
+    with pytest.raises(ValueError, match="editor_type"):
+        guide_descs([GuideSpec("A" * 20, "NGG", 2, "XBE")])
