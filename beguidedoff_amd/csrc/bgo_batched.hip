@@ -41,6 +41,10 @@ constexpr int kMaxFam = 8;     // PAM families per kernel launch
 constexpr int kUnroll = 8;     // guides per compare step
 constexpr int kDirectMax = 8;  // guides of a family compared in place (VALU engine)
 constexpr int kDirectSparse = 4;  // default limit for PAMs passing < 5% of positions
+// Records a wave of an MFMA engine stages in LDS before one atomic reserves
+// their slots (same-address device atomics serialise at ~12 ns each, and a
+// 1.5 M-site scan would otherwise take one per candidate event).
+constexpr uint32_t kRecStage = 64;
 constexpr int kMaxBlock = 256; // threads per block (4 waves)
 constexpr uint32_t kWin = 0xFFFFFu;
 // Work items are handed out by kWorkCtrs counters per launch group, each in a
@@ -88,6 +92,8 @@ struct Tab {
   bgo_hit* __restrict__ hits;
   unsigned long long* __restrict__ hit_count;
   const int4* __restrict__ btab;         // MFMA engines: guide operand tiles
+  bgo_hit* rec_stage;                    // MFMA engines: the wave's staged records (LDS), else null
+  uint32_t* rec_cnt;                     // records staged (LDS)
 };
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
@@ -106,7 +112,8 @@ __host__ __device__ constexpr uint32_t ring_size(int engine) { return ring_flush
 // Bytes of LDS one wave uses with nf families (see the kernel's layout).
 __host__ __device__ constexpr size_t wave_lds_bytes(int nf, int engine) {
   return static_cast<size_t>(nf) * ring_size(engine) * 8 + kMaxFam * 8 +
-         static_cast<size_t>(nf) * 64 * 4 + kMaxFam * 4 + 64 * 8;
+         static_cast<size_t>(nf) * 64 * 4 + kMaxFam * 4 + 64 * 8 +
+         (engine >= 2 ? kRecStage * sizeof(bgo_hit) + 16 : 0);
 }
 
 __device__ inline uint32_t slot_pos(uint2 e) { return (e.x >> 20) | ((e.y >> 20) << 12); }
@@ -117,36 +124,12 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Exact path for a (window, guide) pair whose 2-bit mismatch count c is within
-// the guide's budget: add the N correction (cuda_search.py:290-308), re-test,
-// and append a fully scored record.
-__device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s, uint32_t gi,
-                                         uint32_t pos, uint32_t W0, uint32_t W1, uint32_t c,
-                                         bool cand, uint32_t lane) {
+// The fully scored record of a site (the finder's per-hit epilogue,
+// finder.py:435-470): window and PAM bit planes, 2-bit mismatch mask, CCTOP
+// score and the editable flag.
+__device__ inline bgo_hit make_record(const Args& a, const Tab& t, uint32_t s, uint32_t gi,
+                                      uint32_t pos, uint32_t W0, uint32_t W1, int nmm) {
   const uint32_t g0 = t.g01[2 * gi], g1 = t.g01[2 * gi + 1];
-  const int k = t.gk[gi];
-  int nmm = static_cast<int>(c);
-  bool rec = false;
-  if (cand) {
-    const uint32_t* nw = s ? a.nwords1 : a.nwords0;
-    if (nw) {
-      const uint32_t w = pos >> 5, sh = pos & 31u;
-      const uint32_t nb = funnel(nw[w + 1], nw[w], sh) & kWin;
-      if (nb) nmm += __popc(nb & t.gnm[2 * gi + s]);
-    }
-    rec = nmm <= k;
-  }
-  const uint64_t bal = ballot(rec);
-  if (!bal) return;
-  const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(bal));
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(t.hit_count, static_cast<unsigned long long>(__popcll(bal)));
-  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(base), leader);
-  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(base >> 32), leader);
-  base = (static_cast<unsigned long long>(hi) << 32) | lo;
-  if (!rec) return;
-  const unsigned long long idx = base + lanes_below(bal);
-  if (idx >= a.hit_cap) return;
   const uint2* planes = s ? a.planes1 : a.planes0;
   const uint32_t w = pos >> 5, sh = pos & 31u;
   const uint2 qa = planes[w], qb = planes[w + 1];
@@ -171,7 +154,72 @@ __device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s
     sc += t.score_tab[i + 1];
   }
   h.score = sc;
-  t.hits[idx] = h;
+  return h;
+}
+
+
+// Append the wave's staged records (all lanes call this): one atomic reserves
+// their slots.  Records past the buffer's capacity are counted, not stored.
+__device__ inline void flush_records(const Args& a, const Tab& t, uint32_t lane) {
+  wave_sync();
+  const uint32_t n = uniform(*t.rec_cnt);
+  if (n == 0) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(t.hit_count, static_cast<unsigned long long>(n));
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(base), 0);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(base >> 32), 0);
+  base = (static_cast<unsigned long long>(hi) << 32) | lo;
+  if (lane < n && base + lane < a.hit_cap) t.hits[base + lane] = t.rec_stage[lane];
+  wave_sync();
+  if (lane == 0) *t.rec_cnt = 0;
+  wave_sync();
+}
+
+// Exact path for a (window, guide) pair whose 2-bit mismatch count c is within
+// the guide's budget: add the N correction (cuda_search.py:290-308), re-test,
+// and append a fully scored record.  All lanes of the wave call this.  The
+// MFMA engines stage records in LDS (kRecStage per wave); the VALU engine
+// reserves slots with one atomic per call.
+__device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s, uint32_t gi,
+                                         uint32_t pos, uint32_t W0, uint32_t W1, uint32_t c,
+                                         bool cand, uint32_t lane) {
+  const int k = t.gk[gi];
+  int nmm = static_cast<int>(c);
+  bool rec = false;
+  if (cand) {
+    const uint32_t* nw = s ? a.nwords1 : a.nwords0;
+    if (nw) {
+      const uint32_t w = pos >> 5, sh = pos & 31u;
+      const uint32_t nb = funnel(nw[w + 1], nw[w], sh) & kWin;
+      if (nb) nmm += __popc(nb & t.gnm[2 * gi + s]);
+    }
+    rec = nmm <= k;
+  }
+  const uint64_t bal = ballot(rec);
+  if (!bal) return;
+  if (t.rec_stage) {
+    const uint32_t n = static_cast<uint32_t>(__popcll(bal));
+    uint32_t cnt = uniform(*t.rec_cnt);
+    if (cnt + n > kRecStage) {
+      flush_records(a, t, lane);
+      cnt = 0;
+    }
+    if (rec) t.rec_stage[cnt + lanes_below(bal)] = make_record(a, t, s, gi, pos, W0, W1, nmm);
+    wave_sync();
+    if (lane == 0) *t.rec_cnt = cnt + n;
+    wave_sync();
+    return;
+  }
+  const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(bal));
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(t.hit_count, static_cast<unsigned long long>(__popcll(bal)));
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(base), leader);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(base >> 32), leader);
+  base = (static_cast<unsigned long long>(hi) << 32) | lo;
+  if (!rec) return;
+  const unsigned long long idx = base + lanes_below(bal);
+  if (idx >= a.hit_cap) return;
+  t.hits[idx] = make_record(a, t, s, gi, pos, W0, W1, nmm);
 }
 
 // Compare the first n (<= 64) windows of a family ring with all guides of
@@ -765,7 +813,6 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
     const double* __restrict__ score_tab, bgo_hit* __restrict__ hits,
     unsigned long long* __restrict__ hit_count, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ fam_windows, const int4* __restrict__ btab) {
-  const Tab t{fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count, btab};
   extern __shared__ uint2 lds[];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
@@ -781,7 +828,17 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
   w.pws = reinterpret_cast<uint32_t*>(my + kRings + kMaxFam * 8);
   w.counts = w.pws + NF * 64;
   w.stage = reinterpret_cast<uint2*>(w.counts + kMaxFam);
+  // MFMA engines: kRecStage staged records and their count after the plane words
+  bgo_hit* const rec_stage = ENGINE >= 2 ? reinterpret_cast<bgo_hit*>(w.stage + 64) : nullptr;
+  uint32_t* const rec_cnt =
+      ENGINE >= 2 ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(w.stage + 64) +
+                                                kRecStage * sizeof(bgo_hit))
+                  : nullptr;
+  const Tab t{fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count, btab, rec_stage, rec_cnt};
   if (lane < kMaxFam) w.wstats[lane] = 0;
+  if constexpr (ENGINE >= 2) {
+    if (lane == 0) *rec_cnt = 0;
+  }
   wave_sync();
   // Counter k hands out items k, k + kWorkCtrs, k + 2 kWorkCtrs, ...  Global
   // wave g takes from counter (g + g / kWorkCtrs) % kWorkCtrs: each aligned
@@ -815,6 +872,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
     const uint32_t ce = min(cs + a.chunk, a.valid_end);
     scan_run<NF, ENGINE>(a, t, w, s, cs, ce, lane);
   }
+  if constexpr (ENGINE >= 2) flush_records(a, t, lane);
   wave_sync();
   if (lane < NF && w.wstats[lane]) atomicAdd(&fam_windows[lane], w.wstats[lane]);
 }
