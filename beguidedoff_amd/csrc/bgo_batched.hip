@@ -157,7 +157,6 @@ __device__ inline bgo_hit make_record(const Args& a, const Tab& t, uint32_t s, u
   return h;
 }
 
-
 // Append the wave's staged records (all lanes call this): one atomic reserves
 // their slots.  Records past the buffer's capacity are counted, not stored.
 __device__ inline void flush_records(const Args& a, const Tab& t, uint32_t lane) {
