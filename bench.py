@@ -75,7 +75,7 @@ def mfma_ops(families) -> int:
 # counters' reading for a copy kernel of known size (profiles/
 # batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md).  Keyed by (workload,
 # engine); None when not profiled.
-TRAFFIC_BYTES = {("hg38_mixed", 3): 1_601_617_440}
+TRAFFIC_BYTES = {("hg38_mixed", 3): 1_601_686_768}
 
 
 def copy_probe(mbytes: int) -> None:
