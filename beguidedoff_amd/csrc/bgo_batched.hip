@@ -654,7 +654,12 @@ __device__ inline void flush_front(const Args& a, const Tab& t, const WaveState&
   if constexpr (ENGINE >= 2)
     compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, kFlush, cb, lane);
   else
+  {
+    // VALU batch compare at wave priority 0, like the fp4 full-batch compare
+    __builtin_amdgcn_s_setprio(0);
     compare_list(a, t, t.fam[f], s, L, kFlush, cb, lane);
+    __builtin_amdgcn_s_setprio(1);
+  }
   const uint2 e = L[kFlush + lane];  // LDS operations of a wave complete in order
   L[lane] = e;
   if (lane == 0) w.wstats[f] += kFlush;
@@ -804,7 +809,11 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
       if constexpr (ENGINE >= 2)
         compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane);
       else
+      {
+        __builtin_amdgcn_s_setprio(0);
         compare_list(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane);
+        __builtin_amdgcn_s_setprio(1);
+      }
       if (lane == 0) w.wstats[f] += c;
     }
   }
@@ -841,8 +850,11 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
                                                 kRecStage * sizeof(bgo_hit))
                   : nullptr;
   const Tab t{fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count, btab, rec_stage, rec_cnt};
-  // fp4: priority 1 outside the full-batch compare (set to 0 there)
-  if constexpr (ENGINE == 3) __builtin_amdgcn_s_setprio(1);
+  // fp4 and VALU: priority 1 outside the batch compares, which run at 0 (the
+  // scan, append and in-place passes are dependent chains; a compare has
+  // many independent steps in flight).  VALU: 22 / 80 / 1024 mixed guides
+  // 3.38 / 5.83 / 37.8 -> 3.27 / 5.43 / 35.8 ms.
+  if constexpr (ENGINE == 3 || ENGINE == 1) __builtin_amdgcn_s_setprio(1);
   if (lane < kMaxFam) w.wstats[lane] = 0;
   if constexpr (ENGINE >= 2) {
     if (lane == 0) *rec_cnt = 0;
