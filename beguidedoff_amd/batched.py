@@ -278,13 +278,33 @@ def default_capacity(n_guides: int) -> int:
     return max(1 << 20, 4096 * n_guides)
 
 
+# Most site records one launch may hold (32 B each: 512 MB of device memory
+# and as much page-locked host memory).  Bounds the exact-size re-run of a
+# launch whose sites overflowed its first buffer.
+DEFAULT_MAX_RECORDS = 1 << 24
+
+
+class RecordLimitExceeded(RuntimeError):
+    """A launch found more sites than the caller's record limit allows."""
+
+    def __init__(self, n_sites: int, limit: int):
+        super().__init__(f"{n_sites} sites exceed the record limit of {limit}")
+        self.n_sites = int(n_sites)
+        self.limit = int(limit)
+
+
 def scan(genome: DeviceGenome, specs: Sequence[GuideSpec], tuning: Optional[dict] = None,
          hit_capacity: Optional[int] = None, pos_begin: int = 0,
          pos_end: Optional[int] = None, strands: int = 3,
-         end_guard: Optional[int] = None) -> ScanResult:
+         end_guard: Optional[int] = None,
+         max_records: Optional[int] = DEFAULT_MAX_RECORDS) -> ScanResult:
     """Scan and return every site; re-runs once with an exact-size buffer when
-    the first buffer was too small (the kernel counts past its capacity)."""
+    the first buffer was too small (the kernel counts past its capacity).
+    Raises RecordLimitExceeded when the sites exceed `max_records` (None: no
+    limit); :func:`scan_capped` keeps such scans within a limit."""
     cap = int(hit_capacity or default_capacity(len(specs)))
+    if max_records is not None:
+        cap = min(cap, int(max_records))
     launches = 0
     while True:
         plan = Plan(genome, specs, tuning=tuning, hit_capacity=cap)
@@ -297,6 +317,135 @@ def scan(genome: DeviceGenome, specs: Sequence[GuideSpec], tuning: Optional[dict
             if n <= cap:
                 return ScanResult(plan.fetch(n), n, ms, plan.tuning(), plan.family_stats(),
                                   launches)
+            if max_records is not None and n > max_records:
+                raise RecordLimitExceeded(n, max_records)
             cap = n
         finally:
             plan.close()
+
+
+@dataclass
+class CappedScan:
+    hits: np.ndarray      # kept records, unordered; `guide` indexes the caller's specs
+    counts: np.ndarray    # int64[n_guides, 2]: exact sites per guide on '+' and '-'
+    launches: int
+    kernel_ms: float      # summed over launches
+
+    @property
+    def truncated(self) -> np.ndarray:
+        """Per guide: more sites on a strand than were kept."""
+        kept = np.zeros_like(self.counts)
+        if self.hits.size:
+            np.add.at(kept, (self.hits["guide"].astype(np.int64),
+                             self.hits["strand"].astype(np.int64)), 1)
+        return (self.counts > kept).any(axis=1)
+
+
+def _keep_lowest(hits: np.ndarray, cap: int) -> np.ndarray:
+    """Records of each (guide, strand) with the `cap` lowest positions."""
+    if hits.size == 0:
+        return hits
+    grp = hits["guide"].astype(np.int64) * 2 + hits["strand"].astype(np.int64)
+    order = np.lexsort((hits["pos"], grp))
+    hits, grp = hits[order], grp[order]
+    first = np.r_[True, grp[1:] != grp[:-1]]
+    start = np.maximum.accumulate(np.where(first, np.arange(grp.size), 0))
+    return hits[np.arange(grp.size) - start < cap]
+
+
+def scan_capped(genome: DeviceGenome, specs: Sequence[GuideSpec], cap_per_strand: int,
+                tuning: Optional[dict] = None, strands: int = 3,
+                max_records: int = DEFAULT_MAX_RECORDS) -> CappedScan:
+    """Every site of every guide, except that a guide keeps only its
+    `cap_per_strand` lowest-position sites on a strand (positions in the
+    scanned strand array), with exact per-strand counts.  No launch holds more
+    than `max_records` records: a guide set whose sites exceed it is split in
+    halves, and a single guide whose sites exceed it is scanned per strand
+    over position ranges, lowest first, until `cap_per_strand` sites are
+    kept.  The reference bounds the same work by its per-strand ceiling
+    (finder.py:568-638) and its decode budget of 4,000,000 records
+    (finder.py:116, 157-165)."""
+    cap_per_strand = int(cap_per_strand)
+    max_records = max(int(max_records), 1024)
+    specs = list(specs)
+    n = len(specs)
+    counts = np.zeros((n, 2), dtype=np.int64)
+    parts: List[np.ndarray] = []
+    stat = {"launches": 0, "ms": 0.0}
+
+    def launch(plan, b, e, mask):
+        plan.launch(b, e, mask)
+        m, ms = plan.wait()
+        stat["launches"] += 1
+        stat["ms"] += ms
+        return m
+
+    def fetch_exact(plan, b, e, mask):
+        """All records of [b, e), or None when they exceed max_records."""
+        m = launch(plan, b, e, mask)
+        if m > plan.hit_capacity:
+            if m > max_records:
+                return None
+            plan.set_hit_capacity(m)
+            m = launch(plan, b, e, mask)
+        return plan.fetch(m)
+
+    def group(idx: np.ndarray) -> bool:
+        sub = [specs[i] for i in idx.tolist()]
+        plan = Plan(genome, sub, tuning=tuning,
+                    hit_capacity=min(max_records, default_capacity(len(sub))))
+        try:
+            h = fetch_exact(plan, 0, None, strands)
+        finally:
+            plan.close()
+        if h is None:
+            return False
+        h["guide"] = idx[h["guide"].astype(np.int64)]
+        np.add.at(counts, (h["guide"].astype(np.int64), h["strand"].astype(np.int64)), 1)
+        parts.append(_keep_lowest(h, cap_per_strand))
+        return True
+
+    def single(i: int) -> None:
+        plan = Plan(genome, [specs[i]], tuning=tuning, hit_capacity=max_records)
+        try:
+            for s in (0, 1):
+                if not strands & (1 << s):
+                    continue
+                total = launch(plan, 0, None, 1 << s)
+                counts[i, s] = total
+                kept: List[np.ndarray] = []
+                if total <= max_records:
+                    kept.append(plan.fetch(total))
+                else:  # lowest positions first, ranges sized to the record limit
+                    end = genome.total_bp
+                    lo, got = 0, 0
+                    step = max(64, end // 64)
+                    while lo < end and got < cap_per_strand:
+                        hi = min(end, lo + step)
+                        m = launch(plan, lo, hi, 1 << s)
+                        if m > max_records:  # (64 positions hold <= 64 sites)
+                            step = max(64, (hi - lo) // 2)
+                            continue
+                        kept.append(plan.fetch(m))
+                        got += m
+                        lo = hi
+                        if 4 * m < max_records:
+                            step *= 2
+                h = np.concatenate(kept) if kept else np.zeros(0, dtype=HIT_DTYPE)
+                h["guide"] = i
+                parts.append(_keep_lowest(h, cap_per_strand))
+        finally:
+            plan.close()
+
+    pending = [np.arange(n, dtype=np.int64)] if n else []
+    while pending:
+        idx = pending.pop()
+        if group(idx):
+            continue
+        if idx.size > 1:
+            half = idx.size // 2
+            pending += [idx[half:], idx[:half]]
+        else:
+            single(int(idx[0]))
+    hits = np.concatenate(parts) if parts else np.zeros(0, dtype=HIT_DTYPE)
+    return CappedScan(hits, counts, stat["launches"], stat["ms"])

@@ -226,12 +226,26 @@ extern "C" int bgo_strand_create(const uint32_t* words, int64_t n_words, const u
 
 extern "C" int bgo_strand_destroy(bgo_strand* s) {
   if (!s) return BGO_OK;
-  if (s->d_words) (void)hipFree(s->d_words);
-  if (s->d_nmask) (void)hipFree(s->d_nmask);
-  if (s->d_scratch) (void)hipFree(s->d_scratch);
-  if (s->ev0) (void)hipEventDestroy(s->ev0);
-  if (s->ev1) (void)hipEventDestroy(s->ev1);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  {
+    // A scan running on another thread holds s->mu for its whole call
+    // (uploads, kernels, copies back): the device memory is released only
+    // after it returns.  Calling destroy while other threads may still start
+    // scans on the handle is the caller's error; the Python driver releases a
+    // strand only when no call holds a reference to it.
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->d_words) (void)hipFree(s->d_words);
+    if (s->d_nmask) (void)hipFree(s->d_nmask);
+    if (s->d_scratch) (void)hipFree(s->d_scratch);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->d_words = nullptr;
+    s->d_nmask = nullptr;
+    s->d_scratch = nullptr;
+    s->ev0 = s->ev1 = nullptr;
+    s->stream = nullptr;
+  }
   delete s;
   return BGO_OK;
 }

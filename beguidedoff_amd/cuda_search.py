@@ -115,7 +115,11 @@ class DeviceStrand:
 
 # Upload cache keyed by the identity of the host arrays, like the
 # reference's _GENOME_DEVICE_CACHE (cuda_search.py:321-362), so overflow
-# re-scans do not re-upload the genome.
+# re-scans do not re-upload the genome.  Eviction only drops the cache's
+# references, as the reference's does: a strand that a running call still
+# holds stays alive, and its device memory is released when the last
+# reference goes (DeviceStrand.__del__ -> bgo_strand_destroy, which also
+# waits for a scan in progress on the handle).
 _CACHE: Dict[Tuple[int, int], Tuple[np.ndarray, Optional[np.ndarray], DeviceStrand]] = {}
 _CACHE_LIMIT = 4
 _CACHE_LOCK = threading.Lock()
@@ -128,19 +132,17 @@ def _device_strand(genome_packed, total_bp, n_mask_packed) -> DeviceStrand:
         if (hit is not None and hit[0] is genome_packed and hit[1] is n_mask_packed
                 and hit[2].total_bp == int(total_bp)):
             return hit[2]
-        strand = DeviceStrand(genome_packed, total_bp, n_mask_packed)
         if len(_CACHE) >= _CACHE_LIMIT:
-            for _, (_, _, old) in list(_CACHE.items()):
-                old.close()
             _CACHE.clear()
+        strand = DeviceStrand(genome_packed, total_bp, n_mask_packed)
         _CACHE[key] = (genome_packed, n_mask_packed, strand)
         return strand
 
 
 def clear_device_cache() -> None:
+    """Drop the cached uploads; strands still held by a running call are
+    released when that call returns."""
     with _CACHE_LOCK:
-        for _, (_, _, s) in list(_CACHE.items()):
-            s.close()
         _CACHE.clear()
 
 

@@ -150,6 +150,10 @@ def _guide_planes_2bit(seq: str) -> int:
 class beguidedoff:
     """Extensive (exhaustive GPU) off-target search."""
 
+    # Most site records one batched launch may hold (None: the library's
+    # default, batched.DEFAULT_MAX_RECORDS).
+    max_records: Optional[int] = None
+
     def __init__(self, genome_path: Optional[str] = None, max_mismatches: int = 4,
                  scoring_method: str = "CCTOP", device: Optional[str] = None,
                  n_top_offtargets: int = 3, results_per_guide: int = 10240,
@@ -417,28 +421,24 @@ class beguidedoff:
         Results are exact; a key whose sites on one strand exceed the
         escalation ceiling keeps the lowest-position `ceiling` of them and is
         reported as truncated (the reference keeps the first `ceiling` the
-        kernel stored, finder.py:609-625)."""
-        from .batched import GuideSpec, scan
+        kernel stored, finder.py:609-625).  No launch holds more than
+        `max_records` site records (batched.scan_capped), so a heavy
+        multi-mapper costs bounded device and host memory, as the reference's
+        per-strand ceiling bounds it."""
+        from .batched import GuideSpec, scan_capped
 
         specs = [GuideSpec(sg.guide_seq, sg.pam, self.max_mismatches) for sg in scan_guides]
-        res = scan(self._device_genome(), specs, tuning=self.tuning)
         cap = (self.max_results_per_guide if self.max_results_per_guide is not None
                else _MAX_RESULTS_PER_GUIDE)
+        kw = {} if self.max_records is None else {"max_records": self.max_records}
+        res = scan_capped(self._device_genome(), specs, cap, tuning=self.tuning, **kw)
         hits = res.hits
         key = hits["guide"].astype(np.int64)
         n_keys = len(scan_guides)
-        counts_k = np.bincount(key, minlength=n_keys)
-        grp = key * 2 + hits["strand"].astype(np.int64)
-        sizes = np.bincount(grp, minlength=2 * n_keys)
-        over_k = (sizes.reshape(n_keys, 2) > cap).any(axis=1)
+        counts_k = res.counts.sum(axis=1)
+        over_k = (res.counts > cap).any(axis=1)
         if over_k.any():
-            # per (key, strand) group in position order: keep the first `cap`
-            order = np.lexsort((hits["pos"], grp))
-            hits, key, grp = hits[order], key[order], grp[order]
-            rank = np.arange(grp.size, dtype=np.int64) - (np.cumsum(sizes) - sizes)[grp]
-            keep = rank < cap
-            kept_k = np.bincount(key[keep], minlength=n_keys)
-            hits, key = hits[keep], key[keep]
+            kept_k = np.bincount(key, minlength=n_keys)
             for k in np.flatnonzero(over_k):
                 _warn_truncated(log, scan_guides[k], int(counts_k[k]), cap, int(kept_k[k]))
         flat = _AllHits(key, hits["strand"].astype(np.int8), hits["pos"].astype(np.int64),

@@ -337,6 +337,18 @@ def workload(name: str, seed: int = 0, scale: float = 1.0) -> Workload:
         guides = mixed_guides(rng, 512, 256, 128, 128)
         genome = make_genome(contigs, seed=seed, n_runs=n_runs)
         return Workload(name, genome, guides, 5, "hg38-like 3.09 Gbp, 1024 mixed-PAM guides")
+    if name == "hg38_mixed_planted":
+        # the hg38_mixed guides and background, plus 8 near-copies (0-5
+        # substitutions, a random allowed PAM) of every 8th guide
+        guides = mixed_guides(rng, 512, 256, 128, 128)
+        sel = list(range(0, len(guides), 8))
+        plants = plan_plants(rng, offsets, lengths, *n_runs,
+                             [(guides[i].guide_seq, guides[i].pam) for i in sel], 8, 5)
+        for p in plants:
+            p.guide = sel[p.guide]
+        genome = make_genome(contigs, seed=seed, n_runs=n_runs, plants=plants)
+        return Workload(name, genome, guides, 5,
+                        "hg38-like 3.09 Gbp, 1024 mixed-PAM guides, 8 plants of every 8th guide")
     if name == "hg38_8k":
         seqs = random_guides(rng, 8192)
         guides = _guide_inputs(seqs, ["NGG"] * 8192, [("ABE", 4, 8)] * 8192)

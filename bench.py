@@ -20,6 +20,7 @@ seconds, both strands always scanned.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -71,11 +72,60 @@ def mfma_ops(families) -> int:
 
 
 # HBM traffic of one launch of the dominant kernel on the default workload,
-# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the same
-# counters' reading for a copy kernel of known size (profiles/
-# batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md).  Keyed by (workload,
-# engine); None when not profiled.
-TRAFFIC_BYTES = {("hg38_mixed", 3): 1_601_686_768}
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this benchmark's own
+# step loop, each scaled by the same counter's reading for a copy kernel of
+# known size.  Keyed by (workload, engine, max_mm): (bytes, the profile that
+# holds the counter runs and their commands).  None when not profiled.
+TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_601_686_768,
+                                        "profiles/batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md")}
+
+# Busy fractions of the units of the dominant kernel from rocprofv3 --pmc
+# (SQ_ACTIVE_INST_VALU, SQ_INST_CYCLES_VMEM..., SQ_VALU_MFMA_BUSY_CYCLES) on
+# the same workload, same profile as the traffic.
+UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.88, "mfma": 0.468}}
+
+
+def site_keys(guide, strand, pos, n_mm) -> np.ndarray:
+    """(guide, strand, position, N-corrected mismatches) of each site packed
+    in one uint64, for set comparisons and a hash of the site set."""
+    return ((np.asarray(guide, np.uint64) << np.uint64(38))
+            | (np.asarray(strand, np.uint64) << np.uint64(37))
+            | (np.asarray(pos, np.uint64) << np.uint64(5)) | np.asarray(n_mm, np.uint64))
+
+
+def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, families,
+                   survey_ops, L, kms, peaks) -> dict:
+    """The dominant kernel against its peak, plus what binds it and how the
+    per-unit models of the survey price the same launch."""
+    traffic = TRAFFIC_BYTES.get((workload, engine, max_mm))
+    busy = UNIT_BUSY.get((workload, engine, max_mm))
+    pairs = int(sum(gg * ww for gg, ww in families))
+    secs = kms / 1e3
+    valu_peak = peaks["valu_lane_ops"]
+    out = {"bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": unit,
+           "frac": round(achieved / peak, 4),
+           "traffic": traffic[0] if traffic else None,
+           "traffic_source": traffic[1] if traffic else None,
+           "kernel": "batched_scan_kernel", "ops_per_launch": int(alg),
+           "pairs_per_launch": pairs,
+           "survey_ops_per_launch": int(survey_ops),
+           "hbm_gbps_planes": round(2 * L / 4 / secs / 1e9, 1)}
+    if engine >= 2:
+        out["binding_unit"] = "VALU (feeds the MFMA: window operands, sign tests, appends)"
+        if busy:
+            out["unit_busy"] = dict(busy, source=traffic[1] if traffic else None)
+    # The survey's per-unit models of the same launch: 11 INT32 VALU ops per
+    # (guide, window) pair + 3 per window and family, against the INT32 VALU
+    # peak; 160 int8 ops per pair against the dense int8 MFMA peak.  The
+    # bit-plane and MFMA formulations do less work per pair than the first
+    # model assumes, so a frac above 1 there means the model no longer bounds
+    # this engine, not that a peak was exceeded.
+    out["survey_models"] = {
+        "int32_valu": {"ops": int(survey_ops), "peak_tops": round(valu_peak / 2 / 1e12, 1),
+                       "frac": round(survey_ops / secs / (valu_peak / 2), 3)},
+        "int8_onehot_mfma": {"ops": 160 * pairs, "peak_tops": 5000.0,
+                             "frac": round(160 * pairs / secs / 5.0e15, 3)}}
+    return out
 
 
 def copy_probe(mbytes: int) -> None:
@@ -328,46 +378,124 @@ def fast_mode_leg(workload, hits: np.ndarray, threads: int, cutoff: float = 5.0)
             "gpu": gpu_fast}
 
 
-def serving_leg(dg, pool, max_mm: int, rates, duration: float, seed: int = 0) -> dict:
-    """Poisson arrivals of guide-set requests (log-uniform size 1-500 drawn
-    from `pool`) at each rate for `duration` seconds; p50/p99 latency from
-    arrival to complete site records, and sustained guide*Gbp/s."""
+def _request(rng, pool, max_mm):
+    """One request: a log-uniform number (1-500) of distinct guides from `pool`."""
     from beguidedoff_amd.batched import GuideSpec
+
+    n = int(np.exp(rng.uniform(0, np.log(500))))
+    idx = rng.choice(len(pool), size=n, replace=False)
+    return [GuideSpec(pool[j].guide_seq, pool[j].pam, max_mm, pool[j].editor_type,
+                      (pool[j].edit_window_start, pool[j].edit_window_end)) for j in idx]
+
+
+def serving_leg(dg, pool, max_mm: int, rates, duration: float, min_requests: int = 0,
+                seed: int = 0) -> dict:
+    """Poisson arrivals of guide-set requests (log-uniform size 1-500 drawn
+    from `pool`) at each rate for max(duration, min_requests / rate) seconds;
+    p50/p99 latency from arrival to complete site records, offered
+    guide*Gbp/s and the GPU's busy fraction."""
     from beguidedoff_amd.serving import ScanServer, latency_stats
 
     rng = np.random.default_rng(seed)
     gbp = dg.total_bp / 1e9
     out = {}
     for rate in rates:
+        span = max(duration, min_requests / rate)
         with ScanServer(dg, n_segments=8, hit_budget=1 << 24, pipeline=2) as srv:
             t0 = time.perf_counter()
             handles = []
             t_next = t0
+            t_log = t0 + 30.0
             while True:
                 t_next += rng.exponential(1.0 / rate)
-                if t_next - t0 > duration:
+                if t_next - t0 > span:
                     break
                 delay = t_next - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-                n = int(np.exp(rng.uniform(0, np.log(500))))
-                idx = rng.choice(len(pool), size=n, replace=False)
-                specs = [GuideSpec(pool[j].guide_seq, pool[j].pam, max_mm, pool[j].editor_type,
-                                   (pool[j].edit_window_start, pool[j].edit_window_end))
-                         for j in idx]
-                handles.append(srv.submit(specs))
+                handles.append(srv.submit(_request(rng, pool, max_mm)))
+                if time.perf_counter() > t_log:
+                    done = sum(1 for h in handles if h.done.is_set())
+                    log(f"serving {rate} req/s: {time.perf_counter() - t0:.0f} s, "
+                        f"{len(handles)} submitted, {done} done")
+                    t_log += 30.0
             for h in handles:
                 h.result(timeout=600)
             wall = time.perf_counter() - t0
             st = latency_stats(handles)
             guides = sum(len(h.specs) for h in handles)
             st.update({"rate_req_s": rate, "requests": len(handles), "guides": guides,
-                       "duration_s": round(wall, 2), "offered_s": duration,
+                       "duration_s": round(wall, 2), "offered_s": round(span, 1),
                        "guide_gbp_s": round(guides * gbp / wall, 1),
                        "launches": srv.launches, "reruns": srv.reruns,
                        "gpu_busy_frac": round(srv.busy_s / wall, 3)})
             out[str(rate)] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}
             log(f"serving {rate} req/s: {out[str(rate)]}")
+    return out
+
+
+def serving_capacity(dg, pool, max_mm: int, duration: float, outstanding: int = 32,
+                     seed: int = 1) -> dict:
+    """Closed loop: `outstanding` requests (same size distribution) are kept in
+    the server for `duration` seconds, each replaced as it completes.  The
+    completed requests per second and guide*Gbp/s are the server's capacity
+    at that concurrency; latency then includes queueing."""
+    from beguidedoff_amd.serving import ScanServer, latency_stats
+
+    rng = np.random.default_rng(seed)
+    gbp = dg.total_bp / 1e9
+    with ScanServer(dg, n_segments=8, hit_budget=1 << 24, pipeline=2) as srv:
+        live = [srv.submit(_request(rng, pool, max_mm)) for _ in range(outstanding)]
+        done = []
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < duration:
+            live[0].done.wait(1.0)
+            still = []
+            for h in live:
+                if h.done.is_set():
+                    h.result(0)
+                    done.append(h)
+                    still.append(srv.submit(_request(rng, pool, max_mm)))
+                else:
+                    still.append(h)
+            live = still
+        wall = time.perf_counter() - t0
+        busy = srv.busy_s
+        for h in live:
+            h.result(timeout=600)
+    st = latency_stats(done)
+    guides = sum(len(h.specs) for h in done)
+    st.update({"outstanding": outstanding, "requests": len(done),
+               "req_s": round(len(done) / wall, 1), "guide_gbp_s": round(guides * gbp / wall, 1),
+               "gpu_busy_frac": round(min(1.0, busy / wall), 3), "duration_s": round(wall, 1)})
+    out = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}
+    log(f"serving capacity: {out}")
+    return out
+
+
+def finder_leg(w, max_mm: int, reps: int = 3) -> dict:
+    """End to end through the drop-in API: beguidedoff(mode='extensive')
+    .search(guides=...) for every guide of the workload (dedup, both strands,
+    scoring, editability, top 3 per guide, the 12-column table), median of
+    `reps` calls after one warm call (the genome is resident after the
+    first).  The reference publishes search() times for 100 and 500 guides
+    at <= 4 mm, top 3 (benchmarks/REPORT.md:331-335; RTX 3060, real hg38)."""
+    from beguidedoff_amd import beguidedoff
+
+    f = beguidedoff(mode="extensive", genome=w.genome, max_mismatches=max_mm, n_top_offtargets=3)
+    t0 = time.perf_counter()
+    df = f.search(guides=w.guides, verbose=False)
+    first = time.perf_counter() - t0
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        df = f.search(guides=w.guides, verbose=False)
+        times.append(time.perf_counter() - t0)
+    f.close()
+    out = {"guides": len(w.guides), "rows": int(len(df)), "search_s": round(float(np.median(times)), 4),
+           "first_call_s": round(first, 3),
+           "reference_search_s": {"100 guides": 15.76, "500 guides": 180.42}}
+    log(f"finder: {out}")
     return out
 
 
@@ -387,7 +515,15 @@ def main() -> int:
     ap.add_argument("--eager", action="store_true", help="launch without the hipGraph")
     ap.add_argument("--rates", default="2,8,32", help="serving arrival rates (req/s); '' skips")
     ap.add_argument("--serving-duration", type=float, default=15.0,
-                    help="seconds of Poisson arrivals per rate")
+                    help="seconds of Poisson arrivals per rate (at least)")
+    ap.add_argument("--serving-requests", type=int, default=120,
+                    help="arrivals per rate at least (the window grows to requests / rate)")
+    ap.add_argument("--capacity-s", type=float, default=10.0,
+                    help="seconds of the closed-loop serving capacity run (0 skips)")
+    ap.add_argument("--no-finder", action="store_true",
+                    help="skip the end-to-end finder search (API call, table of the top 3)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving legs (long runs of config 5)")
     ap.add_argument("--no-engines", action="store_true", help="skip the VALU/MFMA comparison")
     ap.add_argument("--guides", type=int, default=0,
                     help="scan a fixed random subset of this many of the workload's guides "
@@ -443,6 +579,16 @@ def main() -> int:
     t0 = time.perf_counter()
     dg = DeviceGenome(g, g["n_mask_le_fwd"], g["n_mask_le_rc"])
     log(f"genome resident in HBM ({dg.device_bytes() / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s")
+    if args.serving_only:
+        rates = [float(r) for r in args.rates.split(",") if r]
+        out = {"metric": "serving latency (config 5)", "workload": w.name, "max_mismatches": args.max_mm,
+               "serving": serving_leg(dg, w.guides, args.max_mm, rates, args.serving_duration,
+                                      args.serving_requests)}
+        if args.capacity_s > 0:
+            out["serving_capacity"] = serving_capacity(dg, w.guides, args.max_mm, args.capacity_s)
+        dg.close()
+        print(json.dumps(out), flush=True)
+        return 0
     tune_trials = None
     if args.tuning == "auto":
         from beguidedoff_amd.autotune import autotune
@@ -485,15 +631,15 @@ def main() -> int:
     ops, survey_ops = algorithmic_ops(L_pos, 2, families)
     engine = plan.tuning()["engine"]
     if engine == 2:  # int8 MFMA compare: priced against the dense int8 MFMA peak
-        bound, unit, dtype = "mfma", "TOP/s", "i8"
+        bound, unit, dtype = "mfma", "TOP/s", "int (exact); int8 MFMA operands"
         alg = mfma_ops(families)
         peak = 5.0e3
     elif engine == 3:  # fp4 MFMA compare: priced against the dense fp4 MFMA peak
-        bound, unit, dtype = "mfma", "TFLOP/s", "fp4"
+        bound, unit, dtype = "mfma", "TFLOP/s", "int (exact); fp4 e2m1 MFMA operands"
         alg = mfma_ops(families)
         peak = 10.1e3
     else:            # VALU compare: 32-bit lane-ops against the VALU issue peak
-        bound, unit, dtype = "valu", "Tops/s", "u32"
+        bound, unit, dtype = "valu", "Tops/s", "u32 (exact)"
         alg = ops
         peak = peaks["valu_lane_ops"] / 1e12
     achieved = alg / (kms / 1e3) / 1e12
@@ -529,13 +675,8 @@ def main() -> int:
         "engines": engines,
         "autotune": tune_trials,
         "kernel_ms": round(kms, 3),
-        "roofline": {"bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 3),
-                     "unit": unit, "frac": round(achieved / peak, 4),
-                     "traffic": TRAFFIC_BYTES.get((w.name, engine)),
-                     "kernel": "batched_scan_kernel", "ops_per_launch": int(alg),
-                     "pairs_per_launch": int(sum(gg * ww for gg, ww in families)),
-                     "survey_ops_per_launch": int(survey_ops),
-                     "hbm_gbps_planes": round(2 * L / 4 / (kms / 1e3) / 1e9, 1)},
+        "roofline": roofline_block(bound, achieved, peak, unit, w.name, engine, args.max_mm,
+                                   alg, families, survey_ops, L, kms, peaks),
     }
 
     if not args.no_plain:
@@ -551,13 +692,21 @@ def main() -> int:
         plain_ms, plain_wall, plain_sites = [], [], 0
         for rep in range(args.plain_steps):
             kt, tw, sites = 0.0, time.perf_counter(), 0
-            for key, nm, enc in (("genome_forward", "n_mask_le_fwd", 0),
-                                 ("genome_rc", "n_mask_le_rc", 3)):
+            plain_keys = []
+            for s_idx, (key, nm, enc) in enumerate((("genome_forward", "n_mask_le_fwd", 0),
+                                                    ("genome_rc", "n_mask_le_rc", 3))):
                 res, ovf, cnt = search_genome_cuda(g[key], L, vals, masks, lens, tp, args.max_mm,
                                                    return_overflow=True, n_mask_packed=g[nm],
                                                    n_encoded=enc)
                 kt += last_kernel_ms()
                 sites += sum(cnt.values())
+                if any(ovf.values()):
+                    raise RuntimeError("plain kernel result buffer overflowed")
+                for gi, lst in res.items():
+                    if lst:
+                        a = np.array(lst, dtype=np.int64)
+                        plain_keys.append(site_keys(np.full(len(lst), gi), np.full(len(lst), s_idx),
+                                                    a[:, 0], a[:, 1]))
             plain_wall.append(time.perf_counter() - tw)
             plain_ms.append(kt)
             plain_sites = sites
@@ -567,11 +716,20 @@ def main() -> int:
                         "wall_s": round(float(np.mean(plain_wall)), 2), "sites": int(plain_sites),
                         "steps": args.plain_steps}
         out["speedup_vs_plain"] = round(pk / kms, 2)
-        out["sites_equal_plain"] = bool(plain_sites == n)
+        pk_keys = np.sort(np.concatenate(plain_keys)) if plain_keys else np.zeros(0, np.uint64)
+        bk_keys = np.sort(site_keys(hits["guide"], hits["strand"], hits["pos"], hits["n_mm"]))
+        out["sites_equal_plain"] = bool(np.array_equal(pk_keys, bk_keys))
+        out["sites_sha256"] = hashlib.sha256(bk_keys.tobytes()).hexdigest()[:16]
 
     if args.rates:
         rates = [float(r) for r in args.rates.split(",") if r]
-        out["serving"] = serving_leg(dg, w.guides, args.max_mm, rates, args.serving_duration)
+        out["serving"] = serving_leg(dg, w.guides, args.max_mm, rates, args.serving_duration,
+                                     args.serving_requests)
+    if args.capacity_s > 0:
+        out["serving_capacity"] = serving_capacity(dg, w.guides, args.max_mm, args.capacity_s)
+
+    if not args.no_finder and args.guides == 0:
+        out["finder"] = finder_leg(w, args.max_mm)
 
     if not args.no_cpu:
         t0 = time.perf_counter()

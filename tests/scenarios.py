@@ -124,3 +124,31 @@ def threshold_genome():
 def threshold_guide():
     return GuideInput("g1", GUIDE, "NGG", "3prime", 4, 8, "CBE", on_target_chrom="chr1",
                       on_target_pos=1, on_target_strand="+")
+
+
+TIE_GUIDE = "GATTCAGCTAGGCTCATGCA"
+
+
+def chrom_name_tie():
+    """Twelve contigs chr1..chr12 in numeric order, each with an exact copy of
+    TIE_GUIDE + AGG at local 1-based position 100 + 7k (chr k), plus a second
+    exact copy on chr10 at 1,500.  Every copy scores 0, so the order of the
+    rows is decided by the chromosome name as a string, then the position
+    (finder.py:552): chr1, chr10, chr10, chr11, chr12, chr2, ..., chr9.
+    Returns (contigs, expected [(chromosome, position)] of the 0-score rows in
+    table order)."""
+    rng = np.random.default_rng(52)
+    contigs, zero = [], []
+    for k in range(1, 13):
+        seq = list(rng.choice(list("ACGT"), size=2400))
+        sites = [100 + 7 * k] + ([1500] if k == 10 else [])
+        for p in sites:
+            seq[p - 4:p - 1] = list("TTT")
+            seq[p - 1:p + 22] = list(TIE_GUIDE + "AGG")
+            zero.append((f"chr{k}", p))
+        contigs.append((f"chr{k}", "".join(seq)))
+    return contigs, sorted(zero)
+
+
+def tie_guide():
+    return GuideInput("tie", TIE_GUIDE, "NGG", "3prime", 4, 8, "CBE")

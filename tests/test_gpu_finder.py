@@ -132,3 +132,51 @@ def test_tsv_input_and_output(gpu, tmp_path):
     assert len(df) > 0 and set(df["base_editor"]) == {"ABE8e"}
     back = pd.read_csv(out, sep="\t")
     assert len(back) == len(df) and list(back.columns) == list(df.columns)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_multimapper_truncation_is_bounded(gpu, engine, monkeypatch, caplog):
+    """A poly-A guide over A/T blocks at <= 5 mm with a ceiling of 1,000 sites
+    per strand: the table keeps exactly the ceiling per strand, the warning
+    carries the exact genome-wide count, and (batched engine) no launch holds
+    more than the record limit."""
+    from beguidedoff_amd import batched as B
+    from test_gpu_kernels import block_genome, oracle_set
+
+    genome = synth.genome_from_strings(block_genome())
+    want = oracle_set(genome, [("A" * 20, "NNN")], 5)
+    per_strand = [sum(1 for h in want if h[1] == s) for s in (0, 1)]
+    assert min(per_strand) > 1000
+    monkeypatch.setattr(finder_module, "_MAX_RESULTS_PER_GUIDE", 1000)
+    fetched = []
+    orig = B.Plan.fetch
+    monkeypatch.setattr(B.Plan, "fetch", lambda self, n=None: fetched.append(
+        (n if n is not None else self.hit_capacity)) or orig(self, n))
+    f = finder(genome, engine, max_mismatches=5, results_per_guide=1000)
+    f.max_records = 4096
+    guides = [GuideInput("polyA", "A" * 20, "NNN", "3prime", 4, 8, "ABE"),
+              GuideInput("other", "ACGTTGCAACGTTGCAACGT", "NGG", "3prime", 4, 8, "ABE")]
+    with caplog.at_level(logging.WARNING):
+        df = f.search(guides=guides, return_all=True, verbose=False)
+    rows = df[df["guide_rna_sequence"] == "A" * 20]
+    assert len(rows) == 2000
+    assert (rows["ot_strand"] == "+").sum() == 1000
+    msgs = [r.getMessage() for r in caplog.records]
+    assert any("Truncated off-target enumeration" in m and str(len(want)) in m for m in msgs)
+    if engine == "batched":
+        assert fetched and max(fetched) <= 4096
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_chromosome_name_tie_break(gpu, engine):
+    """Equal scores are ordered by the chromosome name as a string
+    (finder.py:552): chr1, chr10, chr10, chr11, ... on twelve contigs."""
+    contigs, zero = S.chrom_name_tie()
+    f = finder(contigs, engine)
+    df = f.search(guides=[S.tie_guide()], verbose=False)
+    assert list(zip(df["ot_chromosome"], df["ot_position"])) == zero[:3]
+    df = f.search(guides=[S.tie_guide()], return_all=True, verbose=False)
+    z = df[df["score"] == 0]
+    assert list(zip(z["ot_chromosome"], z["ot_position"])) == zero
+    want = FinderOracle(synth.genome_from_strings(contigs)).search([S.tie_guide()], return_all=True)
+    pd.testing.assert_frame_equal(df, want)

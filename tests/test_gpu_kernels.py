@@ -379,3 +379,77 @@ def test_auto_engine_choice(gpu):
     assert engine_for({"NGG": 36, "NNNRRT": 21, "NNGRRT": 16, "NG": 7}) == 1
     assert engine_for({"NGG": 64, "NNNRRT": 64, "NNGRRT": 64, "NG": 64}) == 3
     assert engine_for({"NGG": 512, "NNNRRT": 256, "NNGRRT": 128, "NG": 128}) == 3
+
+
+def block_genome(seed=3, blocks=24, block_len=1500):
+    """Random contigs interleaved with poly-A and poly-T blocks and N runs: a
+    poly-A guide has tens of thousands of sites on each strand (poly-T blocks
+    read as poly-A on the reverse-complement array)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i in range(blocks):
+        parts.append("".join(rng.choice(list("ACGT"), size=int(rng.integers(1000, 3000)))))
+        parts.append(("A" if i % 2 else "T") * block_len)
+        if i % 5 == 0:
+            parts.append("N" * int(rng.integers(200, 900)))
+    seq = "".join(parts)
+    cut = len(seq) // 3
+    return [("c0", seq[:cut]), ("c1", seq[cut:])]
+
+
+def keep_lowest_set(sites, cap):
+    by = {}
+    for g, s, p, m in sites:
+        by.setdefault((g, s), []).append((p, m))
+    out = set()
+    for (g, s), v in by.items():
+        out |= {(g, s, p, m) for p, m in sorted(v)[:cap]}
+    return out
+
+
+@pytest.mark.parametrize("engine", [pytest.param({"engine": 1}, id="valu"),
+                                    pytest.param({"engine": 3}, id="fp4")])
+def test_scan_capped_bounds_records_and_keeps_lowest(gpu, monkeypatch, engine):
+    """A heavy multi-mapper (poly-A, PAM NNN, <= 5 mm) among ordinary guides:
+    with a record limit of 1024 the guide set is split until the multi-mapper
+    is alone, whose strands are then scanned over position ranges; no fetch
+    exceeds the limit, the per-strand counts are exact and each (guide,
+    strand) keeps its lowest-position `cap` sites, all against the oracle."""
+    from beguidedoff_amd import batched as B
+
+    contigs = block_genome()
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    rng = np.random.default_rng(8)
+    guides = [("".join(rng.choice(list("ACGT"), size=20)), p) for p in ("NGG", "NG", "NNNRRT") * 3]
+    guides.insert(4, ("A" * 20, "NNN"))
+    guides.append(("AAAAAAAAAAAAAAAAAAAC", "NG"))
+    specs = [GuideSpec(s, p, 5) for s, p in guides]
+    want = oracle_set(genome, guides, 5)
+    per = {}
+    for g, s, _, _ in want:
+        per[(g, s)] = per.get((g, s), 0) + 1
+    assert per[(4, 0)] > 20_000 and per[(4, 1)] > 20_000
+    fetched = []
+    orig = B.Plan.fetch
+
+    def spy(self, n=None):
+        out = orig(self, n)
+        fetched.append(out.size)
+        return out
+
+    monkeypatch.setattr(B.Plan, "fetch", spy)
+    cap = 700
+    r = B.scan_capped(dg, specs, cap, tuning=engine, max_records=1024)
+    assert fetched and max(fetched) <= 1024
+    assert hit_set(r.hits) == keep_lowest_set(want, cap)
+    counts = np.zeros((len(guides), 2), np.int64)
+    for (g, s), c in per.items():
+        counts[g, s] = c
+    assert np.array_equal(r.counts, counts)
+    assert r.truncated.tolist() == [bool((counts[i] > cap).any()) for i in range(len(guides))]
+    # without a cap and limit the same launch sequence returns every site
+    full = B.scan_capped(dg, specs, 1 << 30, tuning=engine)
+    assert hit_set(full.hits) == want and full.launches == 1
+    with pytest.raises(B.RecordLimitExceeded):
+        scan(dg, specs, tuning=engine, max_records=1024)

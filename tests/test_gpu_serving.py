@@ -7,6 +7,7 @@ import pytest
 from beguidedoff_amd import synth
 from beguidedoff_amd.batched import DeviceGenome, GuideSpec, scan
 from beguidedoff_amd.serving import ScanServer, latency_stats
+from test_gpu_kernels import hit_set, oracle_set
 
 pytestmark = pytest.mark.gpu
 
@@ -28,12 +29,14 @@ def served_genome(gpu):
     plants = synth.plan_plants(rng, offs, lens, *runs, [(g.guide_seq, g.pam) for g in pool], 3, 4)
     g = synth.make_genome(contigs, seed=9, n_runs=runs, plants=plants)
     dg = DeviceGenome(g, g["n_mask_le_fwd"], g["n_mask_le_rc"])
-    yield dg, pool
+    yield dg, pool, g
     dg.close()
 
 
 def test_results_same_alone_and_in_mix(served_genome):
-    dg, pool = served_genome
+    """Each request's records equal a scan of it alone (every field) and its
+    sites equal the oracle's (per-guide budgets)."""
+    dg, pool, g = served_genome
     rng = np.random.default_rng(1)
     reqs = []
     for i in range(12):
@@ -56,12 +59,15 @@ def test_results_same_alone_and_in_mix(served_genome):
     for specs, h in zip(reqs, got):
         alone = scan(dg, specs).hits
         assert as_set(h, specs) == as_set(alone, specs)
+        want = oracle_set(g, [(s.seq, s.pam) for s in specs], None,
+                          mm_per=[s.max_mm for s in specs])
+        assert hit_set(h) == want
         total += alone.size
     assert total > 0
 
 
 def test_admission_respects_budget_and_reruns(served_genome):
-    dg, pool = served_genome
+    dg, pool, g = served_genome
     specs = [GuideSpec(g.guide_seq, g.pam, 5) for g in pool[:20]]
     with ScanServer(dg, n_segments=3, hit_budget=2048, pipeline=1, min_reserve=2048) as srv:
         a = srv.submit(specs)
@@ -71,3 +77,4 @@ def test_admission_respects_budget_and_reruns(served_genome):
         assert b.admitted_at >= a.done_at
     assert as_set(ra, specs) == as_set(scan(dg, specs).hits, specs)
     assert as_set(rb, specs[:5]) == as_set(scan(dg, specs[:5]).hits, specs[:5])
+    assert hit_set(ra) == oracle_set(g, [(s.seq, s.pam) for s in specs], 5)

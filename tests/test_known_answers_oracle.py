@@ -100,3 +100,18 @@ def test_search_positions():
 def test_count_score_threshold(thr, count):
     df = F(S.threshold_genome()).search([S.threshold_guide()], count_score_threshold=thr)
     assert len(df) == 3 and df["n_offtargets_below_threshold"].iloc[0] == count
+
+
+def test_chromosome_name_tie_break():
+    """Equal scores are ordered by the chromosome name as a string
+    (finder.py:552): with chr1..chr12 the top 3 are chr1, chr10, chr10, not
+    chr1, chr2, chr3."""
+    contigs, zero = S.chrom_name_tie()
+    assert zero[:4] == [("chr1", 107), ("chr10", 170), ("chr10", 1500), ("chr11", 177)]
+    df = F(contigs).search([S.tie_guide()])
+    assert list(zip(df["ot_chromosome"], df["ot_position"])) == zero[:3]
+    assert (df["score"] == 0).all()
+    df = F(contigs).search([S.tie_guide()], return_all=True)
+    z = df[df["score"] == 0]
+    assert list(zip(z["ot_chromosome"], z["ot_position"])) == zero
+    assert z["ot_rank"].tolist() == list(range(1, 14))
