@@ -407,14 +407,17 @@ def keep_lowest_set(sites, cap):
     return out
 
 
+@pytest.mark.parametrize("cap", [20, 700])
 @pytest.mark.parametrize("engine", [pytest.param({"engine": 1}, id="valu"),
                                     pytest.param({"engine": 3}, id="fp4")])
-def test_scan_capped_bounds_records_and_keeps_lowest(gpu, monkeypatch, engine):
+def test_scan_capped_bounds_records_and_keeps_lowest(gpu, monkeypatch, engine, cap):
     """A heavy multi-mapper (poly-A, PAM NNN, <= 5 mm) among ordinary guides:
     with a record limit of 1024 the guide set is split until the multi-mapper
     is alone, whose strands are then scanned over position ranges; no fetch
     exceeds the limit, the per-strand counts are exact and each (guide,
-    strand) keeps its lowest-position `cap` sites, all against the oracle."""
+    strand) keeps its lowest-position `cap` sites, all against the oracle.
+    The multi-mapper has 17,919 / 17,944 sites on '+' / '-' and another guide
+    25 / 34, so cap 20 also truncates inside a group fetch."""
     from beguidedoff_amd import batched as B
 
     contigs = block_genome()
@@ -429,7 +432,7 @@ def test_scan_capped_bounds_records_and_keeps_lowest(gpu, monkeypatch, engine):
     per = {}
     for g, s, _, _ in want:
         per[(g, s)] = per.get((g, s), 0) + 1
-    assert per[(4, 0)] > 20_000 and per[(4, 1)] > 20_000
+    assert per[(4, 0)] > 10_000 and per[(4, 1)] > 10_000
     fetched = []
     orig = B.Plan.fetch
 
@@ -439,7 +442,6 @@ def test_scan_capped_bounds_records_and_keeps_lowest(gpu, monkeypatch, engine):
         return out
 
     monkeypatch.setattr(B.Plan, "fetch", spy)
-    cap = 700
     r = B.scan_capped(dg, specs, cap, tuning=engine, max_records=1024)
     assert fetched and max(fetched) <= 1024
     assert hit_set(r.hits) == keep_lowest_set(want, cap)
