@@ -334,16 +334,16 @@ class CappedScan:
     @property
     def truncated(self) -> np.ndarray:
         """Per guide: more sites on a strand than were kept."""
-        kept = np.zeros_like(self.counts)
-        if self.hits.size:
-            np.add.at(kept, (self.hits["guide"].astype(np.int64),
-                             self.hits["strand"].astype(np.int64)), 1)
+        n = self.counts.shape[0]
+        grp = self.hits["guide"].astype(np.int64) * 2 + self.hits["strand"].astype(np.int64)
+        kept = np.bincount(grp, minlength=2 * n).reshape(n, 2)
         return (self.counts > kept).any(axis=1)
 
 
-def _keep_lowest(hits: np.ndarray, cap: int) -> np.ndarray:
-    """Records of each (guide, strand) with the `cap` lowest positions."""
-    if hits.size == 0:
+def _keep_lowest(hits: np.ndarray, cap: int, sizes: Optional[np.ndarray] = None) -> np.ndarray:
+    """Records of each (guide, strand) with the `cap` lowest positions
+    (`sizes`, when given, are the records per guide * 2 + strand)."""
+    if hits.size == 0 or (sizes is not None and sizes.max(initial=0) <= cap):
         return hits
     grp = hits["guide"].astype(np.int64) * 2 + hits["strand"].astype(np.int64)
     order = np.lexsort((hits["pos"], grp))
@@ -400,9 +400,11 @@ def scan_capped(genome: DeviceGenome, specs: Sequence[GuideSpec], cap_per_strand
             plan.close()
         if h is None:
             return False
-        h["guide"] = idx[h["guide"].astype(np.int64)]
-        np.add.at(counts, (h["guide"].astype(np.int64), h["strand"].astype(np.int64)), 1)
-        parts.append(_keep_lowest(h, cap_per_strand))
+        g = idx[h["guide"].astype(np.int64)]
+        h["guide"] = g
+        sizes = np.bincount(g * 2 + h["strand"].astype(np.int64), minlength=2 * n)
+        counts[:] += sizes.reshape(n, 2)
+        parts.append(_keep_lowest(h, cap_per_strand, sizes))
         return True
 
     def single(i: int) -> None:
