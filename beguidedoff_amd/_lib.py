@@ -76,6 +76,8 @@ _SIGNATURES = {
     "bgo_plan_family_stats": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P, _P]),
     "bgo_scan_batched": (C.c_int, [_P, _P, C.c_int32, _P, C.POINTER(Tuning), _P, C.c_int64,
                                    C.POINTER(C.c_int64)]),
+    "bgo_scan_batched_noalloc": (C.c_int, [_P, _P, C.c_int32, _P, C.POINTER(Tuning), _P,
+                                           C.c_int64, C.POINTER(C.c_int64)]),
     "bgo_encode_genome": (C.c_int, [C.c_int32, _P, C.c_int64, _P, _P, C.c_int32, _P, _P, _P,
                                     C.POINTER(C.c_float)]),
     "bgo_fast_index_create": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(_P)]),

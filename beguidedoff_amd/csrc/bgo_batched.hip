@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <vector>
 
@@ -987,6 +988,9 @@ struct bgo_plan {
   std::vector<int> group_blocks;     // persistent blocks per launch group
   std::vector<size_t> group_lds;     // dynamic LDS bytes per launch group
   int threads = 0;
+  // Tables and hit buffer in memory the plan does not own (the no-alloc
+  // entry point's scratch array): never grown, never freed.
+  bool ext_mem = false;
   std::mutex mu;
 };
 
@@ -1189,8 +1193,8 @@ int plan_configure_launch(bgo_plan* P) {
 extern "C" int bgo_plan_destroy(bgo_plan* P) {
   if (!P) return BGO_OK;
   if (P->graph) (void)hipGraphExecDestroy(P->graph);
-  if (P->d_mem) (void)hipFree(P->d_mem);
-  if (P->d_hits) (void)hipFree(P->d_hits);
+  if (P->d_mem && !P->ext_mem) (void)hipFree(P->d_mem);
+  if (P->d_hits && !P->ext_mem) (void)hipFree(P->d_hits);
   if (P->h_pinned) (void)hipHostFree(P->h_pinned);
   if (P->ev0) (void)hipEventDestroy(P->ev0);
   if (P->ev1) (void)hipEventDestroy(P->ev1);
@@ -1385,6 +1389,11 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   const size_t o_score = off; off = align256(off + 21 * sizeof(double));
   const size_t o_cnt = off; off = align256(off + P->n_counter_words * 8);
   const size_t o_btab = off; off = align256(off + std::max<size_t>(btab_bytes, 16));
+  if (off > P->mem_bytes && P->ext_mem)
+    return fail(BGO_ERR_ARG,
+                "guide tables need %zu bytes; the no-alloc scratch array holds %zu "
+                "(at most %d guides in at most %d PAM families)",
+                off, P->mem_bytes, BGO_NOALLOC_MAX_GUIDES, BGO_NOALLOC_MAX_FAMILIES);
   if (off > P->mem_bytes) {
     // At least 4 MB (tables of ~16K padded guides): freeing device memory waits
     // for the device's running kernels, which would stall a serving loop
@@ -1713,4 +1722,128 @@ extern "C" int bgo_scan_batched(bgo_genome* G, const bgo_guide_desc* guides, int
   bgo_plan_destroy(P);
   if (rc == BGO_OK) *n_hits_out = n;
   return rc;
+}
+
+// ------------------------------------------------------------ no-alloc scan
+//
+// bgo_scan_batched_noalloc runs bgo_scan_batched's kernels with every device
+// buffer they use (guide tables, family and work counters, hit records) in
+// `scratch`, a file-scope device array of a fixed size, so a call allocates
+// no device or page-locked memory.  Each device that loads the library holds
+// its own copy of the array until the process ends.  The host-side name
+// `scratch` is only a stand-in for that memory: bgo_init resolves the device
+// address once per device with hipGetSymbolAddress, and only that address
+// is used afterwards (kernel arguments, copies).
+
+namespace bgo {
+namespace na {
+
+constexpr size_t kTableBytes = size_t{1} << 20;  // guide tables + counters
+constexpr size_t kHitBytes = static_cast<size_t>(BGO_NOALLOC_MAX_HITS) * sizeof(bgo_hit);
+constexpr size_t kScratchBytes = kTableBytes + kHitBytes;
+constexpr int kMaxDevices = 64;
+
+struct Device {
+  std::mutex mu;        // serialises the calls on this device
+  bool ready = false;
+  char* ptr = nullptr;  // device address of this device's `scratch`
+  bgo_plan* plan = nullptr;  // stream, events and host tables, reused per call
+};
+
+Device& device_state(int device) {
+  static Device devs[kMaxDevices];
+  return devs[device];
+}
+
+}  // namespace na
+}  // namespace bgo
+
+static __device__ __align__(256) unsigned char scratch[bgo::na::kScratchBytes];
+
+namespace bgo {
+
+// Called by bgo_init with `device` current: resolve this device's copy of
+// `scratch` and build the reusable plan (stream and timing events).
+int noalloc_setup(int device) {
+  if (device < 0 || device >= na::kMaxDevices)
+    return fail(BGO_ERR_ARG, "device %d: the no-alloc scan supports devices 0..%d", device,
+                na::kMaxDevices - 1);
+  na::Device& d = na::device_state(device);
+  std::lock_guard<std::mutex> lock(d.mu);
+  if (d.ready) return BGO_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(BGO_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                hipGetErrorString(e));
+  auto* P = new bgo_plan();
+  P->device = device;
+  P->ext_mem = true;
+  P->d_mem = p;
+  P->mem_bytes = na::kTableBytes;
+  P->d_hits = reinterpret_cast<bgo_hit*>(static_cast<char*>(p) + na::kTableBytes);
+  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&P->ev0) != hipSuccess || hipEventCreate(&P->ev1) != hipSuccess) {
+    bgo_plan_destroy(P);
+    return fail(BGO_ERR_HIP, "no-alloc stream/event create on device %d", device);
+  }
+  d.ptr = static_cast<char*>(p);
+  d.plan = P;
+  d.ready = true;
+  return BGO_OK;
+}
+
+}  // namespace bgo
+
+extern "C" int bgo_scan_batched_noalloc(bgo_genome* G, const bgo_guide_desc* guides,
+                                        int32_t n_guides, const double* score_table,
+                                        const bgo_tuning* tuning, bgo_hit* out,
+                                        int64_t out_capacity, int64_t* n_hits_out) {
+  if (!G || !score_table || !n_hits_out || out_capacity < 0 || (out_capacity > 0 && !out))
+    return fail(BGO_ERR_ARG, "bgo_scan_batched_noalloc: bad arguments");
+  if (out_capacity > BGO_NOALLOC_MAX_HITS)
+    return fail(BGO_ERR_ARG,
+                "bgo_scan_batched_noalloc: out_capacity %lld exceeds the scratch array's %d "
+                "records",
+                (long long)out_capacity, BGO_NOALLOC_MAX_HITS);
+  if (n_guides > BGO_NOALLOC_MAX_GUIDES)
+    return fail(BGO_ERR_ARG, "bgo_scan_batched_noalloc: %d guides exceed the limit of %d",
+                n_guides, BGO_NOALLOC_MAX_GUIDES);
+  BGO_TRY(validate_guides(guides, n_guides));
+  {
+    std::set<uint64_t> fams;
+    for (int32_t i = 0; i < n_guides; ++i) fams.insert(fam_key(guides[i]));
+    if (fams.size() > static_cast<size_t>(BGO_NOALLOC_MAX_FAMILIES))
+      return fail(BGO_ERR_ARG, "bgo_scan_batched_noalloc: %zu PAM families exceed the limit of %d",
+                  fams.size(), BGO_NOALLOC_MAX_FAMILIES);
+  }
+  BGO_TRY(injected());
+  if (G->device < 0 || G->device >= na::kMaxDevices || !na::device_state(G->device).ready)
+    return fail(BGO_ERR_ARG, "bgo_scan_batched_noalloc: call bgo_init(%d) first", G->device);
+  na::Device& d = na::device_state(G->device);
+  std::lock_guard<std::mutex> lock(d.mu);
+  bgo_plan* P = d.plan;
+  std::lock_guard<std::mutex> plock(P->mu);
+  BGO_HIP(hipSetDevice(G->device));
+  P->g = G;
+  P->end_guard = 26;
+  P->hit_cap = out_capacity;
+  std::memcpy(P->score, score_table, sizeof(P->score));
+  BGO_TRY(resolve_tuning(P, tuning));
+  // tables into the scratch array (error when they do not fit); the kernel
+  // that follows clears every counter, and the scan clears them again
+  BGO_TRY(plan_set_guides(P, guides, n_guides));
+  BGO_HIP(hipEventRecord(P->ev0, P->stream));
+  BGO_TRY(enqueue_scan(P, 0, G->total_bp, 3));
+  BGO_HIP(hipEventRecord(P->ev1, P->stream));
+  unsigned long long n = 0;
+  BGO_HIP(hipMemcpyAsync(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost, P->stream));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  const int64_t m = std::min<int64_t>(static_cast<int64_t>(n), out_capacity);
+  if (m > 0) {
+    BGO_HIP(hipMemcpyAsync(out, P->d_hits, m * sizeof(bgo_hit), hipMemcpyDeviceToHost, P->stream));
+    BGO_HIP(hipStreamSynchronize(P->stream));
+  }
+  *n_hits_out = static_cast<int64_t>(n);
+  return BGO_OK;
 }

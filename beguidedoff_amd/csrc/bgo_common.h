@@ -127,4 +127,8 @@ __device__ inline uint32_t lanes_below(uint64_t mask, uint32_t plus = 0) {
 
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Per-device set-up of bgo_scan_batched_noalloc (bgo_batched.hip), run by
+// bgo_init with the device current.
+int noalloc_setup(int device);
+
 }  // namespace bgo

@@ -34,7 +34,7 @@ extern "C" int bgo_init(int32_t device) {
   if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
     return bgo::fail(BGO_ERR_NODEV, "device %d is %s; this library is built for gfx950", device,
                      prop.gcnArchName);
-  return BGO_OK;
+  return bgo::noalloc_setup(device);
 }
 
 extern "C" int bgo_device_info(int32_t device, int32_t* cu_count, int32_t* clock_khz,

@@ -252,6 +252,27 @@ int bgo_scan_batched(bgo_genome* g, const bgo_guide_desc* guides,
                      const bgo_tuning* tuning, bgo_hit* out,
                      int64_t out_capacity, int64_t* n_hits_out);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same scan, arguments and results as bgo_scan_batched,
+ * but every device buffer its kernels use (guide tables, counters, hit
+ * records) lives in a fixed array of the library's, 9 MiB per device, held
+ * until the process ends; the call allocates no device or page-locked
+ * memory.  Requires bgo_init(device) of the genome's device first (it
+ * resolves that device's array).  Limits: n_guides <= BGO_NOALLOC_MAX_GUIDES
+ * in <= BGO_NOALLOC_MAX_FAMILIES PAM families, out_capacity <=
+ * BGO_NOALLOC_MAX_HITS; larger requests fail with BGO_ERR_ARG.  Calls on one
+ * device share its array, which keeps the last call's contents: the library
+ * serialises them with a lock, so threads that call at once wait for each
+ * other instead of racing on it. */
+#define BGO_NOALLOC_MAX_GUIDES 4096
+#define BGO_NOALLOC_MAX_FAMILIES 64
+#define BGO_NOALLOC_MAX_HITS 262144
+int bgo_scan_batched_noalloc(bgo_genome* g, const bgo_guide_desc* guides,
+                             int32_t n_guides, const double* score_table,
+                             const bgo_tuning* tuning, bgo_hit* out,
+                             int64_t out_capacity, int64_t* n_hits_out);
+
 /* -------------------------------------------------------- fast-mode index
  * Anchor index of a resident genome for the fast mode; replaces the
  * reference's FM-index (fast/fmindex.py, fast/finder.py:149-159).  Every
