@@ -474,6 +474,29 @@ __device__ inline v16i fp4_mma(const v8i& A, v4i b) {
       v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, zero, 4, 4, 0, 127, 0, 126));
 }
 
+// fp4 product of one A tile with one guide tile added to c, with B scale
+// 2^(scale_b - 127) (the three-field test of the full-batch compare).
+__device__ inline v16f fp4_acc(const v8i& A, v4i b, const v16f& c, int scale_b) {
+  const v8i B = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, c, 4, 4, 0, 127, 0, scale_b);
+}
+
+// Start value and test bits of the three-field accumulator (see the full-batch
+// compare): 2^23 + 64 (1 + 2^7 + 2^14), and bit 6 of each 7-bit field.
+constexpr float kField3Start = 9445440.0f;
+constexpr uint32_t kField3Mask = (1u << 6) | (1u << 13) | (1u << 20);
+
+// OR of a three-field accumulator's 16 elements as 32-bit patterns (one
+// v_or_b32 and seven v_or3_b32).  The whole vector is cast: hipcc 7.2 lowers
+// __builtin_bit_cast of a single element of a float vector to element 0.
+__device__ inline uint32_t or16(const v16f& pf) {
+  const v16i p = __builtin_bit_cast(v16i, pf);
+  uint32_t x = static_cast<uint32_t>(p[0]);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) x |= static_cast<uint32_t>(p[r]);
+  return x;
+}
+
 // AND of a product's 16 elements as 32-bit patterns: the sign bit is clear
 // iff some element is >= 0.
 __device__ inline uint32_t and16(const v16i& p) {
@@ -543,10 +566,18 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
     const uint32_t n_tiles = (F.g_end - F.g_begin) / 32;
     v4i b0 = btile(0), b1 = btile(1);
     if (n == kAT * 32) {
-      // Full flush, every A row a window: the eight products of a pair of
-      // guide tiles are issued back to back and AND-reduced as they complete,
-      // with one ballot per pair; a pair with a non-negative element (a
-      // candidate, rare) is redone per A-tile pair.
+      // Full flush, every A row a window.  Three guide tiles share one
+      // accumulator and one test: the products of an A tile with guide
+      // tiles ti, ti + 1, ti + 2 are chained with B scales 2^0, 2^7, 2^14
+      // (instead of 2^-1), so tile q adds 2 (budget - mm) 128^q to each
+      // element, onto a start value of 2^23 + 64 (1 + 128 + 128^2).  Every
+      // partial sum is an integer in [2^23, 2^24), exact in f32 in any
+      // order of accumulation, so the mantissa holds three 7-bit fields
+      // 64 + 2 (budget - mm) in [16, 104] (padding guides: budget - mm =
+      // -24), and bit 6 of a field is set iff mm <= budget.  One OR over the
+      // 16 elements, masked with those bits, tests 3 x 1024 pairs where a
+      // sign test covered 1024.  A triple with a set bit (rare) is redone per
+      // tile with the sign test.
       uint32_t ti = 0;
       // The kernel's waves run at priority 1 except here: the compare keeps
       // many independent products and reductions in flight, the scan and
@@ -554,46 +585,35 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
       // issue slots first (measured 11.73 -> 11.45 ms; the other way round,
       // 12.23 ms).
       __builtin_amdgcn_s_setprio(0);
-      for (; ti + 1 < n_tiles; ti += 2) {
-        // Issue order is fixed: two products in flight at most, each reduced
-        // while the next one runs (the empty asm keeps each partial AND where
-        // it is), and the operand reloads issued right after the last product
-        // that reads them.
-        v16i p0 = fp4_mma(A[0], b0);
-        v16i p1 = fp4_mma(A[1], b0);
-        uint32_t all = and16(p0);
-        asm volatile("" : "+v"(all));
-        p0 = fp4_mma(A[2], b0);
-        all &= and16(p1);
-        asm volatile("" : "+v"(all));
-        p1 = fp4_mma(A[3], b0);
-        b0 = btile(ti + 2);
-        all &= and16(p0);
-        asm volatile("" : "+v"(all));
-        p0 = fp4_mma(A[0], b1);
-        all &= and16(p1);
-        asm volatile("" : "+v"(all));
-        p1 = fp4_mma(A[1], b1);
-        all &= and16(p0);
-        asm volatile("" : "+v"(all));
-        p0 = fp4_mma(A[2], b1);
-        all &= and16(p1);
-        asm volatile("" : "+v"(all));
-        p1 = fp4_mma(A[3], b1);
-        b1 = btile(ti + 3);
-        all &= and16(p0);
-        all &= and16(p1);
-        if (ballot(static_cast<int>(all) >= 0)) {
-          // the rare pair is redone from reloaded operands (the registers
-          // already hold the next pair), which also keeps its products from
-          // being held live across the common path
+      v4i b2 = btile(2);
+      const v16f k3 = v16f{} + kField3Start;
+      for (; ti + 2 < n_tiles; ti += 3) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int at = 0; at < 4; ++at) {
+          v16f c = fp4_acc(A[at], b0, k3, 127);
+          c = fp4_acc(A[at], b1, c, 134);
+          if (at == 3) b0 = btile(ti + 3);
+          c = fp4_acc(A[at], b2, c, 141);
+          if (at == 3) {
+            b1 = btile(ti + 4);
+            b2 = btile(ti + 5);
+          }
+          any |= or16(c);
+          asm volatile("" : "+v"(any));
+        }
+        if (ballot((any & kField3Mask) != 0)) {
+          // the rare triple is redone from reloaded operands (the registers
+          // already hold the next triple)
           fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti), F.g_begin + 32 * ti);
-          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti + 1),
-                   F.g_begin + 32 * ti + 32);
+          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti + 1), F.g_begin + 32 * ti + 32);
+          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti + 2), F.g_begin + 32 * ti + 64);
         }
       }
       __builtin_amdgcn_s_setprio(1);
+      // the last one or two tiles (b0, b1 hold tiles ti, ti + 1)
       if (ti < n_tiles) fp4_tile(a, t, s, L, n, cs, lane, 4, A, b0, F.g_begin + 32 * ti);
+      if (ti + 1 < n_tiles) fp4_tile(a, t, s, L, n, cs, lane, 4, A, b1, F.g_begin + 32 * ti + 32);
       return;
     }
     for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
