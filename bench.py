@@ -76,13 +76,13 @@ def mfma_ops(families) -> int:
 # step loop, each scaled by the same counter's reading for a copy kernel of
 # known size.  Keyed by (workload, engine, max_mm): (bytes, the profile that
 # holds the counter runs and their commands).  None when not profiled.
-TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_601_686_768,
+TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_666_711_636,
                                         "profiles/batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md")}
 
 # Busy fractions of the units of the dominant kernel from rocprofv3 --pmc
 # (SQ_ACTIVE_INST_VALU, SQ_INST_CYCLES_VMEM..., SQ_VALU_MFMA_BUSY_CYCLES) on
 # the same workload, same profile as the traffic.
-UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.88, "mfma": 0.468}}
+UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.706, "mfma": 0.484}}
 
 
 def site_keys(guide, strand, pos, n_mm) -> np.ndarray:
@@ -110,10 +110,13 @@ def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, f
            "pairs_per_launch": pairs,
            "survey_ops_per_launch": int(survey_ops),
            "hbm_gbps_planes": round(2 * L / 4 / secs / 1e9, 1)}
-    if engine >= 2:
-        out["binding_unit"] = "VALU (feeds the MFMA: window operands, sign tests, appends)"
-        if busy:
-            out["unit_busy"] = dict(busy, source=traffic[1] if traffic else None)
+    if engine >= 2 and busy:
+        top = max(busy, key=busy.get)
+        out["binding_unit"] = (
+            f"{top.upper()} {busy[top]:.0%} busy, none saturated: the scan and append phases "
+            "are chains of dependent VALU/LDS steps at 4 waves per SIMD, and the MFMA pipe "
+            "idles while a wave runs them" if busy[top] < 0.8 else f"{top.upper()}")
+        out["unit_busy"] = dict(busy, source=traffic[1] if traffic else None)
     # The survey's per-unit models of the same launch: 11 INT32 VALU ops per
     # (guide, window) pair + 3 per window and family, against the INT32 VALU
     # peak; 160 int8 ops per pair against the dense int8 MFMA peak.  The
