@@ -20,12 +20,22 @@ HIT_DTYPE = np.dtype([
     ("guide", "<u4"), ("pos", "<u4"), ("x0", "<u4"), ("x1", "<u4"), ("mm_mask", "<u4"),
     ("strand", "u1"), ("n_mm", "u1"), ("editable", "u1"), ("pad", "u1"), ("score", "<f8"),
 ], align=True)
+ASM_KEY_DTYPE = np.dtype([
+    ("g0", "<u4"), ("g1", "<u4"), ("ot_local", "<i8"), ("ot_contig", "<i4"), ("ot_strand", "<i4"),
+    ("first_input", "<u4"), ("n_inputs", "<u4"), ("pam_len", "<u4"), ("pad", "<u4"),
+], align=True)
+ASM_INPUT_DTYPE = np.dtype([("id", "<u4"), ("window", "<u4"), ("target", "<u4"), ("order", "<u4")],
+                           align=True)
 FAST_ANCHOR_DTYPE = np.dtype([
     ("key_lo", "<u4"), ("key_hi", "<u4"), ("a0", "<u4"), ("a1", "<u4"), ("g0", "<u4"),
     ("g1", "<u4"), ("n_code", "<u4"), ("pam_nibs", "<u4"), ("guide", "<u4"), ("strand", "u1"),
     ("pam_len", "u1"), ("cold_first", "u1"), ("pad", "u1"),
 ], align=True)
+ASM_ROW_DTYPE = np.dtype([("hit", HIT_DTYPE), ("input", "<u4"), ("rank", "<u4"),
+                          ("contig", "<u4"), ("local", "<u4")], align=True)
 assert GUIDE_DESC_DTYPE.itemsize == 24 and HIT_DTYPE.itemsize == 32
+assert ASM_KEY_DTYPE.itemsize == 40 and ASM_INPUT_DTYPE.itemsize == 16
+assert ASM_ROW_DTYPE.itemsize == 48
 assert FAST_ANCHOR_DTYPE.itemsize == 40
 
 
@@ -76,8 +86,20 @@ _SIGNATURES = {
     "bgo_plan_family_stats": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P, _P]),
     "bgo_scan_batched": (C.c_int, [_P, _P, C.c_int32, _P, C.POINTER(Tuning), _P, C.c_int64,
                                    C.POINTER(C.c_int64)]),
+    "bgo_plan_assemble": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, C.c_int32, _P, _P, _P,
+                                    C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                    C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64), _P,
+                                    C.POINTER(C.c_int64)]),
     "bgo_scan_batched_noalloc": (C.c_int, [_P, _P, C.c_int32, _P, C.POINTER(Tuning), _P,
                                            C.c_int64, C.POINTER(C.c_int64)]),
+    "bgo_fast_index_offsets": (C.c_int, [_P, _P]),
+    "bgo_fast_index_positions": (C.c_int, [_P, C.c_int64, C.c_int64, _P]),
+    "bgo_sa_text_create": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P, C.c_int32,
+                                     C.POINTER(_P)]),
+    "bgo_sa_text_destroy": (C.c_int, [_P]),
+    "bgo_sa_text_long_runs": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "bgo_sa_text_select": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int32, C.c_int64, _P,
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "bgo_encode_genome": (C.c_int, [C.c_int32, _P, C.c_int64, _P, _P, C.c_int32, _P, _P, _P,
                                     C.POINTER(C.c_float)]),
     "bgo_fast_index_create": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(_P)]),

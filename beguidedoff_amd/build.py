@@ -20,7 +20,7 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["bgo_lib.hip", "bgo_plain.hip", "bgo_batched.hip", "bgo_encode.hip",
-               "bgo_fastidx.hip"]
+               "bgo_fastidx.hip", "bgo_sa_order.hip", "bgo_assemble.hip"]
 HOST_SOURCES = ["bgo_synth.c"]
 
 HIP_LIB = PKG / "libbgo_hip.so"

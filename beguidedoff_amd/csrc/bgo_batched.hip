@@ -1744,6 +1744,28 @@ extern "C" int bgo_scan_batched(bgo_genome* G, const bgo_guide_desc* guides, int
   return rc;
 }
 
+namespace bgo {
+
+// The last launch's records on the device (for the table assembly):
+// stored = min(sites, capacity).  Waits for the launch.
+int plan_device_hits(bgo_plan* P, const bgo_hit** hits, int64_t* stored, int64_t* total,
+                     hipStream_t* stream, int32_t* n_guides) {
+  std::lock_guard<std::mutex> lock(P->mu);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  unsigned long long n = 0;
+  BGO_HIP(hipMemcpyAsync(&n, P->d_counters, sizeof(n), hipMemcpyDeviceToHost, P->stream));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  *hits = P->d_hits;
+  *total = static_cast<int64_t>(n);
+  *stored = std::min<int64_t>(static_cast<int64_t>(n), P->hit_cap);
+  *stream = P->stream;
+  *n_guides = P->n_guides;
+  return BGO_OK;
+}
+
+}  // namespace bgo
+
 // ------------------------------------------------------------ no-alloc scan
 //
 // bgo_scan_batched_noalloc runs bgo_scan_batched's kernels with every device

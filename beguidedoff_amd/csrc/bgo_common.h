@@ -131,4 +131,8 @@ __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfir
 // bgo_init with the device current.
 int noalloc_setup(int device);
 
+// The last launch's site records of a plan, on its device (bgo_batched.hip).
+int plan_device_hits(bgo_plan* P, const bgo_hit** hits, int64_t* stored, int64_t* total,
+                     hipStream_t* stream, int32_t* n_guides);
+
 }  // namespace bgo

@@ -425,6 +425,29 @@ extern "C" int bgo_fast_index_bytes(const bgo_fast_index* X, int64_t* bytes_out)
   return BGO_OK;
 }
 
+extern "C" int bgo_fast_index_offsets(bgo_fast_index* X, uint32_t* out) {
+  BGO_TRY(injected());
+  if (!X || !out) return fail(BGO_ERR_ARG, "null pointer");
+  BGO_HIP(hipSetDevice(X->device));
+  BGO_HIP(hipMemcpy(out, X->d_offsets, (fi::kBuckets + 1) * sizeof(uint32_t),
+                    hipMemcpyDeviceToHost));
+  return BGO_OK;
+}
+
+extern "C" int bgo_fast_index_positions(bgo_fast_index* X, int64_t begin, int64_t end,
+                                        uint32_t* out) {
+  BGO_TRY(injected());
+  if (!X || begin < 0 || end < begin || end > static_cast<int64_t>(X->n_pos) ||
+      (end > begin && !out))
+    return fail(BGO_ERR_ARG, "bgo_fast_index_positions: bad range [%lld, %lld)", (long long)begin,
+                (long long)end);
+  if (end == begin) return BGO_OK;
+  BGO_HIP(hipSetDevice(X->device));
+  BGO_HIP(hipMemcpy(out, X->d_pos + begin, (end - begin) * sizeof(uint32_t),
+                    hipMemcpyDeviceToHost));
+  return BGO_OK;
+}
+
 extern "C" int bgo_fast_index_query(bgo_fast_index* X, const bgo_fast_anchor* anchors,
                                     int32_t n_anchors, double cutoff, const double* score_table,
                                     bgo_hit* out, int64_t out_capacity, int64_t* n_hits_out,

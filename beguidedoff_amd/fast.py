@@ -30,10 +30,15 @@ Two engines find the candidates, with identical results:
   host.
 
 The table is the extensive finder's (same 12 columns, sort and top-N).
-Differences from the reference, by construction: the anchor occurrence cap
-(`max_hits_per_anchor`, first hits in suffix-array order) is not applied, so
-results are complete; the reference's cap is reached only by anchors with more
-than 10,000 genome occurrences.
+
+Anchor occurrence cap (fast/finder.py:149-159, 368-375): like the reference,
+an anchor with more than `max_hits_per_anchor` occurrences keeps only the
+first ones of its suffix-array interval in the reference's FM-index text, and
+the guide is reported with the reference's warning.  The index counts every
+anchor's occurrences; for an anchor over the cap, its occurrences are listed
+from the index and the cap of them whose suffixes come first are selected on
+the host (bgo_sa_text_select), and only candidates anchored at those are
+kept.  `max_hits_per_anchor=0` keeps every occurrence.
 """
 from __future__ import annotations
 
@@ -49,7 +54,7 @@ from .finder import (_AllHits, _W20, _LOGGER, _PROFILES_TSV, assemble_results, b
                      cctop_scores)
 from . import _lib
 from . import genome_converter
-from .encoding import encode_dna_rows, reverse_complement_rows
+from .encoding import encode_dna, encode_dna_rows, reverse_complement_rows
 from .io import load_guides_from_tsv
 from .pam import encode_pam_mask
 from .scoring import cctop_table
@@ -135,6 +140,13 @@ def _planes(codes: np.ndarray) -> Tuple[int, int]:
     sh = np.arange(codes.size, dtype=np.uint32)
     c = codes.astype(np.uint32)
     return int(((c & 1) << sh).sum()), int(((c >> 1) << sh).sum())
+
+
+_RC = str.maketrans("ACGTacgt", "TGCAtgca")
+
+
+def _revcomp(s: str) -> str:
+    return s.translate(_RC)[::-1]
 
 
 def _n_bits20(nmask_le: Optional[np.ndarray], pos: np.ndarray) -> np.ndarray:
@@ -228,6 +240,20 @@ class FastIndexGPU:
         _lib.check(_lib.load().bgo_fast_index_bytes(self._h, C.byref(n)))
         return int(n.value)
 
+    def offsets(self) -> np.ndarray:
+        """The bucket table (uint32[2^24 + 1]) in host memory, copied once."""
+        if getattr(self, "_offsets", None) is None:
+            out = np.empty((1 << 24) + 1, dtype=np.uint32)
+            _lib.check(_lib.load().bgo_fast_index_offsets(self._h, _lib.ptr(out)))
+            self._offsets = out
+        return self._offsets
+
+    def positions(self, begin: int, end: int) -> np.ndarray:
+        out = np.empty(max(0, int(end) - int(begin)), dtype=np.uint32)
+        _lib.check(_lib.load().bgo_fast_index_positions(self._h, int(begin), int(end),
+                                                        _lib.ptr(out)))
+        return out
+
     def query(self, anchors: np.ndarray, cutoff: float, capacity: int = 1 << 16):
         """Candidate records of the anchors and the kernel time (ms); re-runs
         once with an exact-size buffer when the first was too small."""
@@ -253,6 +279,51 @@ class FastIndexGPU:
             self.close()
         except Exception:
             pass
+
+
+class SaText:
+    """A forward genome prepared for anchor suffix-array selection
+    (bgo_sa_text; host memory, the words stay referenced here)."""
+
+    def __init__(self, genome: dict):
+        self._words = np.ascontiguousarray(genome["genome_forward"], dtype=np.uint32)
+        self._offs = np.ascontiguousarray(genome["chrom_offsets"], dtype=np.int64)
+        self._lens = np.ascontiguousarray(genome["chrom_lengths"], dtype=np.int64)
+        h = C.c_void_p()
+        _lib.check(_lib.load().bgo_sa_text_create(
+            _lib.ptr(self._words), self._words.size, int(genome["total_bp"]),
+            _lib.ptr(self._offs), _lib.ptr(self._lens), int(self._offs.size), C.byref(h)))
+        self._h = h
+
+    def select(self, positions: np.ndarray, anchor: str, cap: int) -> Tuple[np.ndarray, int]:
+        """(the anchor occurrences kept under the cap, ascending; the anchor's
+        occurrence count) among `positions`."""
+        pos = np.ascontiguousarray(positions, dtype=np.uint32)
+        codes = np.ascontiguousarray(encode_dna(anchor), dtype=np.uint8)
+        out = np.empty(int(cap), dtype=np.uint32)
+        n_occ, n = C.c_int64(0), C.c_int64(0)
+        _lib.check(_lib.load().bgo_sa_text_select(self._h, _lib.ptr(pos), pos.size,
+                                                  _lib.ptr(codes), codes.size, int(cap),
+                                                  _lib.ptr(out), C.byref(n_occ), C.byref(n)))
+        return out[:n.value], int(n_occ.value)
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.load().bgo_sa_text_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _hits_subset(h: _AllHits, keep: np.ndarray) -> _AllHits:
+    def sub(a):
+        return None if a is None else a[keep]
+    return _AllHits(h.key[keep], h.strand[keep], h.pos[keep], h.n_mm[keep], sub(h.mm_mask),
+                    sub(h.score), sub(h.x0), sub(h.x1), sub(h.mm_out))
 
 
 class beguidedofffast:
@@ -295,6 +366,7 @@ class beguidedofffast:
             raise ValueError(f"engine must be 'index' or 'scan', got {engine!r}")
         self.engine = engine
         self._index: Optional[FastIndexGPU] = None
+        self._sa_text: Optional[SaText] = None
         if engine == "index":
             self._index = FastIndexGPU(self._ext._device_genome(), self._ext.chrom_offsets,
                                        self._ext.chrom_lengths)
@@ -307,12 +379,68 @@ class beguidedofffast:
     def chrom_names(self) -> List[str]:
         return self._ext.chrom_names
 
-    def _candidates(self, scan_guides, cutoff: float) -> _AllHits:
+    def _candidates(self, scan_guides, cutoff: float,
+                    log: Optional[logging.Logger] = None) -> _AllHits:
         """Fast-mode candidates of every scan key, as flat hits whose score,
-        mismatch mask and count include the N positions."""
+        mismatch mask and count include the N positions, under the anchor
+        occurrence cap."""
         if self._index is not None:
-            return self._candidates_index(scan_guides, cutoff)
-        return self._candidates_scan(scan_guides, cutoff)
+            hits = self._candidates_index(scan_guides, cutoff)
+        else:
+            hits = self._candidates_scan(scan_guides, cutoff)
+        return self._anchor_cap(scan_guides, cutoff, hits, log or _LOGGER)
+
+    def _anchor_cap(self, scan_guides, cutoff: float, hits: _AllHits,
+                    log: logging.Logger) -> _AllHits:
+        """Keep only candidates anchored at the first max_hits_per_anchor
+        occurrences of their anchor in the reference's suffix-array order
+        (fast/finder.py:149-159, csrc/fmindex_binding.cpp:164-176), and warn
+        for each guide whose anchor has more (fast/finder.py:368-375)."""
+        cap = self.max_hits_per_anchor
+        if cap <= 0 or not scan_guides:
+            return hits
+        if self._index is None:  # the scan engine counts occurrences with an index too
+            self._index = FastIndexGPU(self._ext._device_genome(), self._ext.chrom_offsets,
+                                       self._ext.chrom_lengths)
+        anchors = anchor_table(scan_guides, cutoff)
+        offs = self._index.offsets()
+        lo = offs[anchors["key_lo"]].astype(np.int64)
+        hi = offs[anchors["key_hi"]].astype(np.int64)
+        over = np.flatnonzero(hi - lo > cap)  # bucket counts bound the occurrence counts
+        if over.size == 0:
+            return hits
+        if self._sa_text is None:
+            self._sa_text = SaText(self._ext.genome_data)
+        c0, m = cold_region(cutoff)
+        ext = self._ext
+        keep = np.ones(hits.key.size, dtype=bool)
+        warned = set()
+        for a in over.tolist():
+            k, strand = a // 2, a % 2
+            sg = scan_guides[k]
+            g = sg.guide_seq.upper()
+            anchor = g[c0:c0 + m] if strand == 0 else _revcomp(g)[:m]
+            kept, n_occ = self._sa_text.select(self._index.positions(lo[a], hi[a]), anchor, cap)
+            if n_occ <= cap:
+                continue
+            if (sg.guide_seq, sg.pam) not in warned:
+                warned.add((sg.guide_seq, sg.pam))
+                log.warning("Anchor hit cap (%d) exceeded for guide %s (PAM=%s); results may "
+                            "be incomplete.", cap, sg.guide_seq, sg.pam)
+            rows = np.flatnonzero((hits.key == k) & (hits.strand == strand))
+            if rows.size == 0:
+                continue
+            pos = hits.pos[rows]
+            plen = len(sg.pam)
+            if strand == 0:  # the '+' anchor starts at the site's cold position c0
+                occ = pos + c0
+            else:  # '-': the forward window's leftmost + PAM length
+                ci = np.searchsorted(ext.chrom_offsets, pos, side="right") - 1
+                left = (ext.chrom_offsets[ci] + ext.chrom_lengths[ci]
+                        - (pos - ext.chrom_offsets[ci]) - (_GUIDE_LENGTH + plen))
+                occ = left + plen
+            keep[rows] = np.isin(occ, kept.astype(np.int64))
+        return _hits_subset(hits, keep) if not keep.all() else hits
 
     def _candidates_index(self, scan_guides, cutoff: float) -> _AllHits:
         anchors = anchor_table(scan_guides, cutoff)
@@ -330,6 +458,9 @@ class beguidedofffast:
         if self._index is not None:
             self._index.close()
             self._index = None
+        if self._sa_text is not None:
+            self._sa_text.close()
+            self._sa_text = None
 
     def _candidates_scan(self, scan_guides, cutoff: float) -> _AllHits:
         from .batched import GuideSpec, scan
@@ -399,7 +530,7 @@ class beguidedofffast:
         if verbose:
             log.info("Fast search: %d unique protospacers (from %d input rows), cutoff=%s",
                      len({(sg.guide_seq, sg.pam) for sg in scan_guides}), len(guides), cutoff)
-        hits = self._candidates(scan_guides, cutoff)
+        hits = self._candidates(scan_guides, cutoff, log)
         df = assemble_results(self._ext, guides, scan_guides, mapping, hits,
                               count_score_threshold, return_all)
         if output_file:

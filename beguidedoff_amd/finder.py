@@ -153,6 +153,9 @@ class beguidedoff:
     # Most site records one batched launch may hold (None: the library's
     # default, batched.DEFAULT_MAX_RECORDS).
     max_records: Optional[int] = None
+    # Batched engine: build the table on the device from the records in HBM
+    # (bgo_plan_assemble); False assembles every record on the host.
+    device_assembly: bool = True
 
     def __init__(self, genome_path: Optional[str] = None, max_mismatches: int = 4,
                  scoring_method: str = "CCTOP", device: Optional[str] = None,
@@ -312,6 +315,15 @@ class beguidedoff:
             log.info("Guides after deduplication: %d (from %d input rows)", len(scan_guides),
                      len(guides))
             log.info("Running GPU search on %s (%s engine) ...", self.device, self.engine)
+        if self.engine == "batched" and self.device_assembly:
+            df = self._search_device_table(guides, scan_guides, mapping, count_score_threshold,
+                                           return_all)
+            if df is not None:
+                if output_file:
+                    df.to_csv(output_file, sep="\t", index=False)
+                    if verbose:
+                        log.info("Wrote results to %s", output_file)
+                return df
         if self.engine == "plain":
             raw, overflow, _counts = self._search_gpu(scan_guides, log, verbose)
         else:
@@ -448,6 +460,149 @@ class beguidedoff:
         overflow = {sg: bool(over_k[k]) for k, sg in enumerate(scan_guides)}
         counts = {sg: int(counts_k[k]) for k, sg in enumerate(scan_guides)}
         return flat, overflow, counts
+
+    def _search_device_table(self, guides, scan_guides, mapping, count_score_threshold,
+                             return_all) -> Optional[pd.DataFrame]:
+        """One launch over every key and the table assembled where the records
+        lie (bgo_plan_assemble): coordinates, on-target exclusion,
+        editability per input, below-threshold counts, per-id order and top
+        N on the device; only the rows come back.  None when the launch's
+        sites overflow its buffer or a key exceeds the escalation ceiling on a
+        strand: the host path then runs (scan_capped + assemble_results)."""
+        from .batched import DEFAULT_MAX_RECORDS, GuideSpec, Plan, default_capacity
+
+        n_keys = len(scan_guides)
+        if n_keys == 0:
+            return None
+        specs = [GuideSpec(sg.guide_seq, sg.pam, self.max_mismatches) for sg in scan_guides]
+        limit = self.max_records if self.max_records is not None else DEFAULT_MAX_RECORDS
+        cap = min(limit, default_capacity(n_keys))
+        plan = Plan(self._device_genome(), specs, tuning=self.tuning, hit_capacity=cap)
+        try:
+            plan.launch()
+            n, _ms = plan.wait()
+            if n > cap:
+                return None
+            keys, inputs, gid, n_ids, input_index = self._assembly_tables(guides, scan_guides,
+                                                                          mapping)
+            ceiling = (self.max_results_per_guide if self.max_results_per_guide is not None
+                       else _MAX_RESULTS_PER_GUIDE)
+            names = self.chrom_names
+            rank_of = {nm: i for i, nm in enumerate(sorted(set(names)))}
+            name_rank = np.array([rank_of[nm] for nm in names], dtype=np.uint32)
+            top = max(int(self.n_top_offtargets), 0)  # <= 0: no rows, as the host path
+            rows_cap = max(1, n_ids * top) if not return_all else max(1, n * len(inputs))
+            below = np.zeros(max(n_ids, 1), dtype=np.int64)
+            while True:
+                rows = np.zeros(rows_cap, dtype=_lib.ASM_ROW_DTYPE)
+                n_rows, mx = C.c_int64(0), C.c_int64(0)
+                _lib.check(_lib.load().bgo_plan_assemble(
+                    plan.handle, _lib.ptr(keys), n_keys, _lib.ptr(inputs), len(inputs), n_ids,
+                    _lib.ptr(self.chrom_offsets), _lib.ptr(self.chrom_lengths),
+                    _lib.ptr(name_rank), len(names), top, int(bool(return_all)),
+                    float(count_score_threshold) if count_score_threshold is not None else 0.0,
+                    int(count_score_threshold is not None), int(ceiling), _lib.ptr(rows), rows_cap,
+                    C.byref(n_rows), _lib.ptr(below), C.byref(mx)))
+                if mx.value > ceiling:
+                    return None
+                if n_rows.value <= rows_cap:
+                    rows = rows[: n_rows.value]
+                    break
+                rows_cap = int(n_rows.value)
+        finally:
+            plan.close()
+        return self._rows_table(guides, scan_guides, rows, gid, below[:n_ids], input_index)
+
+    def _assembly_tables(self, guides, scan_guides, mapping):
+        """bgo_asm_key per scan key and bgo_asm_input per (key, input) pair,
+        key-major with each key's inputs in input order; guide ids numbered
+        by first appearance."""
+        input_index = {id(g): i for i, g in enumerate(guides)}
+        id_of: Dict[str, int] = {}
+        gid = np.array([id_of.setdefault(g.guide_id, len(id_of)) for g in guides], dtype=np.int64)
+        keys = np.zeros(len(scan_guides), dtype=_lib.ASM_KEY_DTYPE)
+        gp0, gp1 = _guide_planes([sg.guide_seq for sg in scan_guides])
+        keys["g0"], keys["g1"] = gp0, gp1
+        keys["pam_len"] = [len(sg.pam) for sg in scan_guides]
+        keys["ot_contig"] = [sg.on_target_chrom_idx if sg.on_target_chrom_idx is not None else -1
+                             for sg in scan_guides]
+        keys["ot_local"] = [sg.on_target_local_pos if sg.on_target_local_pos is not None else 0
+                            for sg in scan_guides]
+        keys["ot_strand"] = [0 if sg.on_target_strand == "+" else
+                             (1 if sg.on_target_strand == "-" else -9) for sg in scan_guides]
+        pairs = [(k, input_index[id(g)]) for k, sg in enumerate(scan_guides) for g in mapping[sg]]
+        counts = np.bincount([k for k, _ in pairs], minlength=len(scan_guides))
+        keys["n_inputs"] = counts
+        keys["first_input"] = np.cumsum(counts) - counts
+        inputs = np.zeros(len(pairs), dtype=_lib.ASM_INPUT_DTYPE)
+        win_of: Dict[tuple, int] = {}
+        for j, (_k, i) in enumerate(pairs):
+            g = guides[i]
+            base = EDITABLE_BASE.get(g.editor_type)
+            if base is None:
+                raise ValueError(f"Unknown editor_type: {g.editor_type}")
+            w = (g.edit_window_start, g.edit_window_end)
+            if w not in win_of:
+                win_of[w] = edit_window_mask(*w)
+            inputs[j] = (gid[i], win_of[w], {"A": 0, "C": 1}[base], j)
+        inp_of_pair = np.array([i for _, i in pairs], dtype=np.int64)
+        return keys, inputs, gid, len(id_of), inp_of_pair
+
+    def _rows_table(self, guides, scan_guides, rows, gid, below, inp_of_pair) -> pd.DataFrame:
+        """The 12-column table from device rows ordered by (id, rank): each
+        input guide contributes its id's rows, in input order
+        (finder.py:545-566)."""
+        if rows.size == 0:
+            return pd.DataFrame()
+        h = rows["hit"]
+        rid = gid[inp_of_pair[rows["input"].astype(np.int64)]]
+        n_ids = int(gid.max()) + 1 if gid.size else 0
+        per_id = np.bincount(rid, minlength=n_ids)
+        starts = np.cumsum(per_id) - per_id
+        sel = np.concatenate([np.arange(starts[gid[i]], starts[gid[i]] + per_id[gid[i]])
+                              for i in range(len(guides))])
+        if sel.size == 0:
+            return pd.DataFrame()
+        r = rows[sel]
+        h = r["hit"]
+        key = h["guide"].astype(np.int64)
+        strand = h["strand"].astype(np.int64)
+        pos = h["pos"].astype(np.int64)
+        x0, x1 = h["x0"].astype(np.uint32), h["x1"].astype(np.uint32)
+        plen_k = np.array([len(sg.pam) for sg in scan_guides], dtype=np.int64)
+        shifts = np.arange(26, dtype=np.uint32)
+        codes = (((x0[:, None] >> shifts) & 1) | (((x1[:, None] >> shifts) & 1) << 1)).astype(
+            np.uint8)
+        for s, _label, words, _nm in self._strand_arrays():  # the rows' bases against the genome
+            m = strand == s
+            if m.any():
+                ref = _codes_at(words, pos[m], 26)
+                width = 20 + plen_k[key[m]]
+                if np.any((ref != codes[m]) & (np.arange(26)[None, :] < width[:, None])):
+                    raise RuntimeError("site bases in the hit records disagree with the genome")
+        gseq_k = np.array([sg.guide_seq for sg in scan_guides], dtype=object)
+        inp = inp_of_pair[r["input"].astype(np.int64)]
+        editor_in = np.array([g.base_editor or "" for g in guides], dtype=object)
+        pam_str = np.empty(sel.size, dtype=object)
+        for pl in np.unique(plen_k[key]):
+            m = plen_k[key] == pl
+            pam_str[m] = _strings(codes[m, 20:20 + pl])
+        cols = {
+            "guide_rna_sequence": gseq_k[key],
+            "base_editor": editor_in[inp],
+            "ot_rank": r["rank"].astype(np.int64) + 1,
+            "ot_chromosome": self._chrom_names_arr[r["contig"].astype(np.int64)],
+            "ot_position": r["local"].astype(np.int64) + 1,
+            "ot_strand": np.where(strand == 0, "+", "-").astype(object),
+            "ot_sequence": _strings(codes[:, :20]),
+            "ot_pam": pam_str,
+            "n_mismatches": h["n_mm"].astype(np.int64),
+            "mismatch_positions": np.array([_mm_string(int(m)) for m in h["mm_mask"]],
+                                           dtype=object),
+            "score": np.array([_round6(float(x)) for x in h["score"]], dtype=np.float64),
+            "n_offtargets_below_threshold": below[gid[inp]],
+        }
+        return pd.DataFrame(cols, columns=OUTPUT_COLUMNS)
 
     def close(self) -> None:
         if self._genome_handle is not None:

@@ -373,6 +373,11 @@ def fast_mode_leg(workload, hits: np.ndarray, threads: int, cutoff: float = 5.0)
     return {"value": round(len(guides) * gbp / qn, 1), "value_1t": round(len(guides) * gbp / q1, 1),
             "unit": "guide·Gbp/s", "cores": threads, "kind": "port",
             "sample": f"all {len(guides)} 20-nt guides, whole genome, cutoff {cutoff}",
+            "algorithm": "the reference's anchored search over a direct-address 10-mer position "
+                         "index (oracle/bgo_fast.c), not its sdsl FM-index locate: faster per "
+                         "query than the algorithm it stands for, whose published time is "
+                         "1.266 s for 100 guides on real hg38 (about 245 guide·Gbp/s, "
+                         "benchmarks/REPORT.md:333); anchor cap in position order",
             "query_s": round(qn, 4), "query_s_1t": round(q1, 4), "index_build_s": round(build_s, 2),
             "index_bytes": index_bytes,
             "verdicts": {"guides": len(guides), "agree": agree, "disagree": disagree,

@@ -244,6 +244,60 @@ int bgo_plan_family_stats(const bgo_plan* p, int32_t cap, int32_t* n_fam_out,
                           int64_t* windows_out /*[cap]*/,
                           int32_t* guides_out /*[cap]*/);
 
+/* ------------------------------------------------ table assembly on the device
+ * The extensive finder's per-hit epilogue and table order
+ * (beguidedoff/finder.py:193-244, 405-472, 545-566) over the last launch's
+ * records where they lie in HBM, so only table rows cross PCIe.  Per record
+ * (site) of scan key k: forward leftmost and reported position (for '-'
+ * records the reverse-complement array is reversed per contig), the
+ * on-target exclusion, and, for each input guide mapped to k, the
+ * editable-base test; rows are ordered per guide id by (score, chromosome
+ * name, reported position, strand, key, input) and cut to the top n_top
+ * unless return_all. */
+typedef struct bgo_asm_key {
+  uint32_t g0, g1;        /* guide planes of the key's first 20 bases (checks mm_mask) */
+  int64_t ot_local;       /* on-target local leftmost (0-based), if ot_contig >= 0 */
+  int32_t ot_contig;      /* on-target contig index; -1: none (finder.py:208-218) */
+  int32_t ot_strand;      /* 0 '+', 1 '-' */
+  uint32_t first_input;   /* the key's inputs: inputs[first_input, + n_inputs) */
+  uint32_t n_inputs;
+  uint32_t pam_len;       /* 1..6 */
+  uint32_t pad;
+} bgo_asm_key;            /* 40 bytes */
+
+typedef struct bgo_asm_input {
+  uint32_t id;            /* guide_id index, numbered by first appearance */
+  uint32_t window;        /* edit window mask: bit i = site base i (0-based) */
+  uint32_t target;        /* editable base: 0 'A' (ABE), 1 'C' (CBE/CGE) */
+  uint32_t order;         /* rank of this (key, input) pair, key-major (< 2^31) */
+} bgo_asm_input;          /* 16 bytes */
+
+typedef struct bgo_asm_row {
+  bgo_hit hit;            /* the site */
+  uint32_t input;         /* index into the inputs array */
+  uint32_t rank;          /* 0-based rank within its guide id */
+  uint32_t contig;        /* contig of the reported position */
+  uint32_t local;         /* 0-based reported position in that contig */
+} bgo_asm_row;            /* 48 bytes */
+
+/* keys[n_keys] (n_keys = the plan's guides, in its guide order); inputs
+ * grouped by key; contig_name_rank[c] = rank of contig c's name among the
+ * sorted distinct names.  threshold is used when use_threshold != 0:
+ * below_out[id] = editable sites with score <= threshold (finder.py:466-470).
+ * When a key has more than strand_ceiling sites on one strand, nothing is
+ * assembled: *max_strand_sites_out > strand_ceiling tells the caller to take
+ * the truncating host path.  Rows come ordered by (id, rank); when there
+ * are more than rows_cap, *n_rows_out is their number and only rows_cap are
+ * stored.  The last launch must have stored all its sites (BGO_ERR_ARG
+ * otherwise). */
+int bgo_plan_assemble(bgo_plan* p, const bgo_asm_key* keys, int32_t n_keys,
+                      const bgo_asm_input* inputs, int32_t n_inputs, int32_t n_ids,
+                      const int64_t* contig_offsets, const int64_t* contig_lengths,
+                      const uint32_t* contig_name_rank, int32_t n_contigs, int32_t n_top,
+                      int32_t return_all, double threshold, int32_t use_threshold,
+                      int64_t strand_ceiling, bgo_asm_row* rows, int64_t rows_cap,
+                      int64_t* n_rows_out, int64_t* below_out, int64_t* max_strand_sites_out);
+
 /* One-shot convenience: create a plan, scan both strands of the whole
  * genome, fetch, destroy.  Returns the exact count in n_hits_out; stores
  * min(n_hits, out_capacity) records. */
@@ -318,6 +372,41 @@ int bgo_fast_index_query(bgo_fast_index* x, const bgo_fast_anchor* anchors,
                          int32_t n_anchors, double cutoff, const double* score_table,
                          bgo_hit* out, int64_t out_capacity, int64_t* n_hits_out,
                          float* kernel_ms);
+
+/* The index's bucket table, copied to host memory: out[key] = first entry
+ * of key's positions, out[2^24] = number of positions (uint32[2^24 + 1]).
+ * With bgo_fast_index_positions it lists an anchor's occurrences, which the
+ * reference's FM-index gives as an SA interval (csrc/fmindex_binding.cpp:
+ * 141-176). */
+int bgo_fast_index_offsets(bgo_fast_index* x, uint32_t* out);
+/* Entries [begin, end) of the index's position array (forward genome
+ * positions grouped by key; order within a key unspecified). */
+int bgo_fast_index_positions(bgo_fast_index* x, int64_t begin, int64_t end, uint32_t* out);
+
+/* --------------------------------------------- anchor suffix-array order
+ * The reference's fast mode keeps the first max_hits_per_anchor entries of
+ * an anchor's suffix-array interval in its FM-index text
+ * (csrc/fmindex_binding.cpp:164-176, fast/finder.py:149-159): contigs as
+ * symbols A C G T = 1 2 3 4 (N and other bytes = 1), a separator 6 between
+ * contigs, a terminal 0 (fast/fmindex.py:63-128, fast/backend.py:28-49).
+ * bgo_sa_text holds a 2-bit forward genome (caller-owned words, which must
+ * outlive it; contigs non-empty and tiling the genome) and its maximal runs
+ * of >= 64 equal bases.  Host code: no GPU is used. */
+typedef struct bgo_sa_text bgo_sa_text;
+int bgo_sa_text_create(const uint32_t* words, int64_t n_words, int64_t total_bp,
+                       const int64_t* contig_offsets, const int64_t* contig_lengths,
+                       int32_t n_contigs, bgo_sa_text** out);
+int bgo_sa_text_destroy(bgo_sa_text* t);
+/* Number of maximal runs of >= 64 equal bases in the genome. */
+int bgo_sa_text_long_runs(const bgo_sa_text* t, int64_t* n_out);
+/* Of `positions`, the occurrences of the m-base anchor (codes 0..3, m <= 32)
+ * that lie inside one contig (n_occ_out of them); if there are more than
+ * cap (>= 1), the cap occurrences whose suffixes of the reference's text are
+ * smallest, else all.  out (capacity cap) receives them in ascending
+ * position order, n_out their number. */
+int bgo_sa_text_select(const bgo_sa_text* t, const uint32_t* positions, int64_t n,
+                       const uint8_t* anchor, int32_t m, int64_t cap, uint32_t* out,
+                       int64_t* n_occ_out, int64_t* n_out);
 
 /* ---------------------------------------------------------- genome encoder
  * FASTA sequence bytes -> genome artefact arrays on `device`; replaces the

@@ -272,9 +272,11 @@ def repeat_case(seed):
 @pytest.mark.gpu
 @pytest.mark.parametrize("cutoff", [5.0, 7.0])
 def test_index_on_repeats_and_n_runs(gpu, cutoff):
+    """Every occurrence kept (max_hits_per_anchor=0): the A-rich anchors have
+    ~64,000 occurrences here, over the reference's default cap."""
     contigs, guides = repeat_case(3)
     genome = synth.genome_from_strings(contigs)
-    f = make_finder(mode="fast", genome=genome, search_cutoff=cutoff)
+    f = make_finder(mode="fast", genome=genome, search_cutoff=cutoff, max_hits_per_anchor=0)
     got = product_candidates(f, contigs, guides, cutoff)
     want = [brute_candidates(contigs, s, p, cutoff) for s, p in guides]
     assert got == want
@@ -305,4 +307,57 @@ def test_index_query_grows_its_buffer_and_sizes(gpu):
     other = FastIndexGPU(ext._device_genome(), ext.chrom_offsets, ext.chrom_lengths)
     assert rows(other.query(anchors, 5.0)[0]) == rows(big)
     other.close()
+    f.close()
+
+
+def cap_case(seed):
+    """A small genome with anchors far over a small cap: N blocks (indexed as
+    A; one ends the last contig, so its suffixes meet the terminal 0), poly-A
+    and poly-T runs, 120 tandem copies of a near-site, random sequence."""
+    rng = np.random.default_rng(seed)
+    unit_guide = "".join(rng.choice(list("ACGT"), size=20))
+    unit = near(rng, unit_guide, 1, 0) + "TGG"
+    r = random_contigs(rng, [2000, 1500, 1000, 800, 1500], n_frac=0.0)
+    contigs = [("chr1", r[0][1] + "N" * 600 + r[1][1] + "A" * 300 + r[2][1]),
+               ("chr2", "A" * 150 + unit * 120 + "T" * 200 + r[3][1]),
+               ("chr3", r[4][1] + "N" * 250)]
+    guides = [(unit_guide, "NGG"), ("A" * 20, "NNN"), ("T" * 20, "NNN"),
+              ("ACGTAC" + "A" * 14, "NGG"), ("GATTACA" + "A" * 13, "NG")]
+    return contigs, guides
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["index", "scan"])
+@pytest.mark.parametrize("cap", [1, 7, 40, 0])
+def test_anchor_cap_keeps_the_first_in_suffix_array_order(gpu, engine, cap, caplog):
+    """The reference keeps the first max_hits_per_anchor occurrences of an
+    anchor's suffix-array interval (csrc/fmindex_binding.cpp:164-176) and
+    warns (fast/finder.py:368-375): the candidates equal the literal brute
+    force restricted to the occurrences a brute-force suffix sort of the
+    reference's index text keeps (oracle/sa_order.py)."""
+    import logging
+
+    from oracle.sa_order import capped_occurrences
+
+    contigs, guides = cap_case(11)
+    genome = synth.genome_from_strings(contigs)
+    f = make_finder(mode="fast", genome=genome, engine=engine, max_hits_per_anchor=cap)
+    with caplog.at_level(logging.WARNING):
+        got = product_candidates(f, contigs, guides, 5.0)
+    c0, m = cold_region(5.0)
+    over = set()
+    for i, ((seq, pam), cand) in enumerate(zip(guides, got)):
+        want = brute_candidates(contigs, seq, pam, 5.0)
+        for strand, anchor in (("+", seq[c0:20]), ("-", revcomp(seq)[:m])):
+            kept, total = capped_occurrences(contigs, anchor, cap)
+            if cap > 0 and total > cap:
+                over.add(seq)
+                boff = c0 if strand == "+" else len(pam)
+                want = {w for w in want if w[1] != strand or w[0] + boff in set(kept)}
+        assert cand == want, (seq, pam, cap)
+    msgs = [r.getMessage() for r in caplog.records if "Anchor hit cap" in r.getMessage()]
+    assert {s for s in over if any(s in x for x in msgs)} == over
+    assert len(msgs) == len(over)
+    if cap in (1, 7):
+        assert len(over) >= 4  # the tandem unit and the A-rich anchors
     f.close()

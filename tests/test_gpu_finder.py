@@ -180,3 +180,80 @@ def test_chromosome_name_tie_break(gpu, engine):
     assert list(zip(z["ot_chromosome"], z["ot_position"])) == zero
     want = FinderOracle(synth.genome_from_strings(contigs)).search([S.tie_guide()], return_all=True)
     pd.testing.assert_frame_equal(df, want)
+
+
+def test_batched_truncation_at_the_reference_ceiling(gpu, caplog):
+    """The reference's ceiling, 2,621,440 sites per strand (finder.py:111): a
+    3 Mbp poly-A contig (then a poly-C one) gives a poly-A guide with PAM NNN
+    about 3 M '+' sites and no '-' sites.  The batched finder keeps the
+    lowest-position 2,621,440, all editable 0-mm sites, so the count below a
+    threshold is exactly the ceiling; the warning carries the exact
+    genome-wide count (the oracle's); the top rows are the first positions."""
+    from test_gpu_kernels import oracle_set
+
+    contigs = [("chrA", "A" * 3_000_000), ("chrB", "C" * 2_000)]
+    genome = synth.genome_from_strings(contigs)
+    want = oracle_set(genome, [("A" * 20, "NNN")], 4)
+    assert all(s == 0 for _, s, _, _ in want) and len(want) > 2_999_000
+    f = finder(genome, "batched", max_mismatches=4)
+    guide = GuideInput("polyA", "A" * 20, "NNN", "3prime", 4, 8, "ABE")
+    with caplog.at_level(logging.WARNING):
+        df = f.search(guides=[guide], count_score_threshold=5.0, verbose=False)
+    assert df["n_offtargets_below_threshold"].tolist() == [2_621_440] * 3
+    assert df["ot_position"].tolist() == [1, 2, 3] and (df["ot_chromosome"] == "chrA").all()
+    msgs = [r.getMessage() for r in caplog.records if "Truncated off-target" in r.getMessage()]
+    assert len(msgs) == 1 and f" {len(want)} raw" in msgs[0] and "keeping the first 2621440" in msgs[0]
+
+
+def assembly_case(seed):
+    """random_case plus: a second input sharing an id with another sequence,
+    an input sharing both id and sequence (a repeated block), an on-target
+    input (its own key), and many contigs with names out of string order."""
+    genome, guides = random_case(seed)
+    g0 = guides[0]
+    guides.append(GuideInput(g0.guide_id, guides[1].guide_seq, guides[1].pam, "3prime", 2, 9, "ABE",
+                             base_editor="shared"))
+    guides.append(GuideInput(guides[2].guide_id, guides[2].guide_seq, guides[2].pam, "3prime",
+                             guides[2].edit_window_start, guides[2].edit_window_end,
+                             guides[2].editor_type))
+    names = [str(n) for n in genome["chrom_names"]]
+    ot = GuideInput("ot", guides[3].guide_seq, guides[3].pam, "3prime", 1, 20, "ABE",
+                    on_target_chrom=names[1], on_target_pos=5, on_target_strand="-")
+    guides.append(ot)
+    return genome, guides
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("kw", [{}, {"return_all": True, "count_score_threshold": 5.0},
+                                {"count_score_threshold": 3.0}, {"n_top": 1}, {"n_top": 0}])
+def test_device_assembly_equals_host_and_oracle(gpu, seed, kw):
+    """The device table (bgo_plan_assemble) equals the host assembly of the
+    same records and the finder oracle, row for row."""
+    kw = dict(kw)
+    top = kw.pop("n_top", 3)
+    genome, guides = assembly_case(seed)
+    dev = finder(genome, "batched", max_mismatches=5, n_top_offtargets=top)
+    host = finder(genome, "batched", max_mismatches=5, n_top_offtargets=top)
+    host.device_assembly = False
+    got = dev.search(guides=guides, verbose=False, **kw)
+    ref = host.search(guides=guides, verbose=False, **kw)
+    want = FinderOracle(genome, max_mismatches=5, n_top_offtargets=top).search(guides, **kw)
+    pd.testing.assert_frame_equal(got, ref)
+    pd.testing.assert_frame_equal(got, want)
+    if top > 0:
+        assert len(got) > 5
+
+
+def test_device_assembly_names_out_of_order(gpu):
+    """Twelve contigs chr1..chr12 and equal scores: the device order follows
+    the name as a string (finder.py:552), with a shared id over two keys."""
+    contigs, zero = S.chrom_name_tie()
+    g = synth.genome_from_strings(contigs)
+    inputs = [S.tie_guide(), GuideInput("tie", "ACGTACGTACGTACGTACGT", "NGG", "3prime", 4, 8, "CBE")]
+    f = finder(g, "batched")
+    df = f.search(guides=inputs, return_all=True, verbose=False)
+    host = finder(g, "batched")
+    host.device_assembly = False
+    pd.testing.assert_frame_equal(df, host.search(guides=inputs, return_all=True, verbose=False))
+    z = df[df["score"] == 0]
+    assert list(zip(z["ot_chromosome"], z["ot_position"]))[:13] == zero
