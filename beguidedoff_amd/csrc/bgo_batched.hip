@@ -439,13 +439,16 @@ __device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const
 #pragma unroll
   for (int r = 0; r < 16; ++r) all &= static_cast<uint32_t>(r1[r]);
   if (!ballot(static_cast<int>(all) >= 0)) return;
-  uint32_t bits = 0;
+  // Sign bits shifted in from the top register down, so bit r (16 + r) is
+  // register r of r0 (r1).  A shift-in chain needs only inline constants;
+  // OR-ing 1 << r per register would keep 32 such constants in VGPRs for the
+  // whole kernel (VOP3 takes no literals here).
+  uint32_t neg = 0;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    bits |= static_cast<uint32_t>(r0[r] >= 0) << r;
-    bits |= static_cast<uint32_t>(r1[r] >= 0) << (16 + r);
-  }
-  mfma_exact<ENGINE>(a, t, s, L, n, cs, lane, bits, gt, abase);
+  for (int r = 15; r >= 0; --r) neg = (neg << 1) | (static_cast<uint32_t>(r1[r]) >> 31);
+#pragma unroll
+  for (int r = 15; r >= 0; --r) neg = (neg << 1) | (static_cast<uint32_t>(r0[r]) >> 31);
+  mfma_exact<ENGINE>(a, t, s, L, n, cs, lane, ~neg, gt, abase);
 }
 
 // int8: one 32-guide tile against the (up to) two 32-window A tiles.
@@ -464,27 +467,37 @@ __device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const ui
   mfma_signs<2>(a, t, s, L, n, cs, lane, acc0, acc1, gt, 0);
 }
 
-// fp4 product of one A tile with one guide tile (scales 2^0), as 32-bit
+// E8M0 scales of the fp4 products, one per byte of a single register (the
+// instruction's op_sel picks the byte): 127 = 2^0 (the A scale and field 0
+// of the three-field test), 134 = 2^7 and 141 = 2^14 (fields 1 and 2), 126 =
+// 2^-1 (the sign test: guide weights are stored doubled).
+constexpr int kScaleBytes = 127 | (134 << 8) | (141 << 16) | (126 << 24);
+
+// fp4 product of one A tile with one guide tile (B scale 2^-1), as 32-bit
 // patterns for the sign test.
 __device__ inline v16i fp4_mma(const v8i& A, v4i b) {
   const v16f zero = {};
   const v8i B = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
-  // B scale 2^-1 (E8M0 126): guide weights are stored doubled
-  return __builtin_bit_cast(
-      v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, zero, 4, 4, 0, 127, 0, 126));
+  return __builtin_bit_cast(v16i, __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                                      A, B, zero, 4, 4, 0, kScaleBytes, 3, kScaleBytes));
 }
 
 // fp4 product of one A tile with one guide tile added to c, with B scale
-// 2^(scale_b - 127) (the three-field test of the full-batch compare).
-__device__ inline v16f fp4_acc(const v8i& A, v4i b, const v16f& c, int scale_b) {
+// 2^(7 Q): field Q of the three-field test of the full-batch compare.
+template <int Q>
+__device__ inline v16f fp4_acc(const v8i& A, v4i b, const v16f& c) {
+  static_assert(Q >= 0 && Q <= 2, "three fields");
   const v8i B = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
-  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, c, 4, 4, 0, 127, 0, scale_b);
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, c, 4, 4, 0, kScaleBytes, Q,
+                                                         kScaleBytes);
 }
 
 // Start value and test bits of the three-field accumulator (see the full-batch
 // compare): 2^23 + 64 (1 + 2^7 + 2^14), and bit 6 of each 7-bit field.
 constexpr float kField3Start = 9445440.0f;
 constexpr uint32_t kField3Mask = (1u << 6) | (1u << 13) | (1u << 20);
+constexpr uint32_t kField2Mask = (1u << 6) | (1u << 13);  // chains of two products
+constexpr uint32_t kField1Mask = 1u << 6;                 // a single product
 
 // OR of a three-field accumulator's 16 elements as 32-bit patterns (one
 // v_or_b32 and seven v_or3_b32).  The whole vector is cast: hipcc 7.2 lowers
@@ -522,6 +535,31 @@ __device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const u
   }
 }
 
+// Three-field test of the products of one or two guide tiles (b0, then b1)
+// with the KAT window tiles, in that order, chained three at a time: field q
+// of a chain is the pair (window 32 at + m, guide n) of its q-th product.  A
+// chain of one or two products tests only its own fields (the others stay
+// at 64).  Returns whether a test bit is set anywhere in the wave.
+template <uint32_t KAT, uint32_t NP>
+__device__ inline bool leftover_chains(const v8i (&A)[4], v4i b0, v4i b1, const v16f& k3) {
+  uint32_t any = 0;
+  v16f c = k3;
+#pragma unroll
+  for (uint32_t p = 0; p < NP; ++p) {
+    const v8i& at = A[p % KAT];
+    const v4i b = p < KAT ? b0 : b1;
+    if (p % 3 == 0) c = fp4_acc<0>(at, b, k3);
+    else if (p % 3 == 1) c = fp4_acc<1>(at, b, c);
+    else c = fp4_acc<2>(at, b, c);
+    if (p % 3 == 2 || p == NP - 1) {
+      const uint32_t mask = p % 3 == 2 ? kField3Mask : p % 3 == 1 ? kField2Mask : kField1Mask;
+      any |= or16(c) & mask;
+      asm volatile("" : "+v"(any));
+    }
+  }
+  return ballot(any != 0) != 0;
+}
+
 // Compare the first n (<= ring_flush(ENGINE)) windows of a family ring with
 // all guides of the family on the matrix cores.  The guide operand tables end
 // with spare all-zero tiles, so the look-ahead loads of later tiles stay in
@@ -533,15 +571,16 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
   constexpr uint32_t kAT = ring_flush(ENGINE) / 32;  // A tiles
   const uint32_t m = lane & 31u, g = lane >> 5;
   if constexpr (ENGINE == 3) {
+    static_assert(kAT >= 1 && kAT <= 4, "fp4_tile takes up to four window tiles");
     // Lane group g's 32 code bits of window row 32 tile + m: the b0 (g = 0)
     // or b1 (g = 1) plane of the 20 bases, then bits 0..11 (g = 0) or
     // 12..19 (g = 1) of b0^b1 at bits 20 and up; bits 28..31 of group 1
     // are the bias elements.
     const uint32_t xmask = g ? 0xFF000u : 0xFFFu, xshift = g ? 8u : 20u;
     const uint32_t pad = g ? 0x40000000u : 0u;  // bias elements = 2.0
-    v8i A[4];
+    v8i A[4] = {};  // tiles kAT.. unused
 #pragma unroll
-    for (int tile = 0; tile < 4; ++tile) {
+    for (int tile = 0; tile < static_cast<int>(kAT); ++tile) {
       const uint32_t widx = 32u * tile + m;
       uint2 e = make_uint2(0u, 0u);
       if (widx < n) e = L[widx];
@@ -590,12 +629,12 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
       for (; ti + 2 < n_tiles; ti += 3) {
         uint32_t any = 0;
 #pragma unroll
-        for (int at = 0; at < 4; ++at) {
-          v16f c = fp4_acc(A[at], b0, k3, 127);
-          c = fp4_acc(A[at], b1, c, 134);
-          if (at == 3) b0 = btile(ti + 3);
-          c = fp4_acc(A[at], b2, c, 141);
-          if (at == 3) {
+        for (int at = 0; at < static_cast<int>(kAT); ++at) {
+          v16f c = fp4_acc<0>(A[at], b0, k3);
+          c = fp4_acc<1>(A[at], b1, c);
+          if (at == static_cast<int>(kAT) - 1) b0 = btile(ti + 3);
+          c = fp4_acc<2>(A[at], b2, c);
+          if (at == static_cast<int>(kAT) - 1) {
             b1 = btile(ti + 4);
             b2 = btile(ti + 5);
           }
@@ -605,15 +644,23 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
         if (ballot((any & kField3Mask) != 0)) {
           // the rare triple is redone from reloaded operands (the registers
           // already hold the next triple)
-          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti), F.g_begin + 32 * ti);
-          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti + 1), F.g_begin + 32 * ti + 32);
-          fp4_tile(a, t, s, L, n, cs, lane, 4, A, btile(ti + 2), F.g_begin + 32 * ti + 64);
+          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, btile(ti), F.g_begin + 32 * ti);
+          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, btile(ti + 1), F.g_begin + 32 * ti + 32);
+          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, btile(ti + 2), F.g_begin + 32 * ti + 64);
+        }
+      }
+      // the last one or two guide tiles (b0, b1 hold tiles ti, ti + 1)
+      const uint32_t left = n_tiles - ti;
+      if (left == 1) {
+        if (leftover_chains<kAT, kAT>(A, b0, b0, k3))
+          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, b0, F.g_begin + 32 * ti);
+      } else if (left == 2) {
+        if (leftover_chains<kAT, 2 * kAT>(A, b0, b1, k3)) {
+          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, b0, F.g_begin + 32 * ti);
+          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, b1, F.g_begin + 32 * ti + 32);
         }
       }
       __builtin_amdgcn_s_setprio(1);
-      // the last one or two tiles (b0, b1 hold tiles ti, ti + 1)
-      if (ti < n_tiles) fp4_tile(a, t, s, L, n, cs, lane, 4, A, b0, F.g_begin + 32 * ti);
-      if (ti + 1 < n_tiles) fp4_tile(a, t, s, L, n, cs, lane, 4, A, b1, F.g_begin + 32 * ti + 32);
       return;
     }
     for (uint32_t ti = 0; ti < n_tiles; ti += 2) {

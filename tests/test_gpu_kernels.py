@@ -290,17 +290,19 @@ def test_budget_and_composition_extremes(gpu, engine):
 
 @pytest.mark.parametrize("engine", ENGINES)
 def test_odd_guide_tile_counts_with_planted_sites(gpu, engine):
-    """Families of 2, 3 and 5 guide tiles (40, 70 and 130 guides): full
-    128-window batches take the fp4 pair path, with the odd last tile alone.
-    Near-copies of guides from every tile are planted, so candidates fall in
-    both tiles of a pair and in the last tile."""
+    """Families of 1, 2, 3, 4 and 5 guide tiles (20, 40, 70, 100 and 130
+    guides).  In full 128-window batches the fp4 engine tests three guide
+    tiles per chain and chains the products of the one or two tiles left
+    over across window tiles (4 tiles: a triple, then one tile; 5: a triple,
+    then two; 1 and 2: leftovers only).  Near-copies of guides from every
+    tile are planted, so candidates fall in every chain position."""
     from brute import IUPAC, plant
     rng = np.random.default_rng(23)
     contigs, _, _ = build_case(23, length=(60_000, 90_000), n_guides=1)
     guides = []
-    for pam, n in (("NG", 40), ("NGG", 70), ("NNNRRT", 130)):
+    for pam, n in (("NG", 40), ("NGG", 70), ("NNNRRT", 130), ("NNGRRT", 100), ("NAG", 20)):
         guides += [("".join(rng.choice(list("ACGT"), size=20)), pam) for _ in range(n)]
-    for gi in rng.choice(len(guides), size=60, replace=False):
+    for gi in rng.choice(len(guides), size=90, replace=False):
         q, pam = guides[int(gi)]
         pam_seq = "".join(rng.choice(list(IUPAC[c])) for c in pam)
         plant(rng, contigs, q, pam_seq, int(rng.integers(0, 4)), "+" if rng.random() < 0.5 else "-")
