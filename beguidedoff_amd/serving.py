@@ -21,6 +21,11 @@ persistent grid, plus the last work items).  On a 3.09 Gbp genome with
 6.2-6.6 ms and p99 of 17.7-19.9 ms, against 6.5-7.2 and 19.1-21.4 ms for 4
 and 6.8-7.6 and 19.3-24.4 ms for 2.
 
+Large requests (`solo_guides`, off by default): requests of at least that
+many guides get launches of their own, taken in turn with the shared launch
+of the smaller requests, so a small request's launches never carry a large
+request's guides.  Each launch group keeps its own segment cursor.
+
 Admission: each request reserves an estimated number of hit records (from
 its PAMs and mismatch budgets); requests are admitted in arrival order while
 the sum of active reservations stays within `hit_budget`.  A segment whose
@@ -77,6 +82,7 @@ class Request:
     launched: int = 0
     retired: int = 0
     admitted_at: float = 0.0
+    seg: int = 0  # next segment of a request scanned in launches of its own
     done_at: float = 0.0
     parts: List[np.ndarray] = field(default_factory=list)
     done: threading.Event = field(default_factory=threading.Event)
@@ -97,9 +103,14 @@ class Request:
 
 class ScanServer:
     def __init__(self, genome: DeviceGenome, n_segments: int = 8, hit_budget: int = 1 << 24,
-                 pipeline: int = 2, tuning: Optional[dict] = None, min_reserve: int = 1024):
+                 pipeline: int = 2, tuning: Optional[dict] = None, min_reserve: int = 1024,
+                 solo_guides: Optional[int] = None):
         if n_segments < 1 or pipeline < 1:
             raise ValueError("n_segments and pipeline must be >= 1")
+        if solo_guides is not None and solo_guides < 1:
+            raise ValueError("solo_guides must be >= 1")
+        self.solo_guides = solo_guides
+        self._turn = 0
         self.genome = genome
         self.n_segments = n_segments
         self.hit_budget = int(hit_budget)
@@ -181,7 +192,7 @@ class ScanServer:
                         self._cv.wait(0.05)
                         continue
                 if launchable and self._free:
-                    inflight.append(self._launch(launchable))
+                    inflight.append(self._launch(*self._next_group(launchable)))
                     continue
                 self._retire(inflight.popleft())
         except BaseException as exc:  # fail every pending request loudly
@@ -193,7 +204,17 @@ class ScanServer:
                 self._queue.clear()
             raise
 
-    def _launch(self, reqs: List[Request]):
+    def _next_group(self, launchable: List[Request]):
+        """The requests of the next launch and whether they are a solo group."""
+        if self.solo_guides is None:
+            return launchable, False
+        shared = [r for r in launchable if len(r.specs) < self.solo_guides]
+        groups = ([(shared, False)] if shared else []) + \
+            [([r], True) for r in launchable if len(r.specs) >= self.solo_guides]
+        self._turn += 1
+        return groups[self._turn % len(groups)]
+
+    def _launch(self, reqs: List[Request], solo: bool = False):
         plan = self._free.popleft()
         specs: List[GuideSpec] = []
         mapping = []
@@ -201,8 +222,12 @@ class ScanServer:
             mapping.append((r, len(specs), len(r.specs)))
             specs.extend(r.specs)
             r.launched += 1
-        seg = self._seg
-        self._seg = (self._seg + 1) % self.n_segments
+        if solo:  # a large request alone walks its own segments
+            seg = reqs[0].seg
+            reqs[0].seg = (seg + 1) % self.n_segments
+        else:
+            seg = self._seg
+            self._seg = (self._seg + 1) % self.n_segments
         cap = max(1 << 12, int(2 * sum(r.reserve for r in reqs) / self.n_segments))
         if cap > plan.hit_capacity:
             plan.set_hit_capacity(max(cap, 2 * plan.hit_capacity))

@@ -33,9 +33,11 @@ def served_genome(gpu):
     dg.close()
 
 
-def test_results_same_alone_and_in_mix(served_genome):
+@pytest.mark.parametrize("solo", [None, 20])
+def test_results_same_alone_and_in_mix(served_genome, solo):
     """Each request's records equal a scan of it alone (every field) and its
-    sites equal the oracle's (per-guide budgets)."""
+    sites equal the oracle's (per-guide budgets); also with requests of 20
+    guides or more in launches of their own."""
     dg, pool, g = served_genome
     rng = np.random.default_rng(1)
     reqs = []
@@ -46,7 +48,8 @@ def test_results_same_alone_and_in_mix(served_genome):
                            pool[j].editor_type, (pool[j].edit_window_start, pool[j].edit_window_end))
                  for j in idx]
         reqs.append(specs)
-    with ScanServer(dg, n_segments=5, hit_budget=1 << 20, pipeline=2) as srv:
+    assert any(len(r) >= 20 for r in reqs) and any(len(r) < 20 for r in reqs)
+    with ScanServer(dg, n_segments=5, hit_budget=1 << 20, pipeline=2, solo_guides=solo) as srv:
         handles = []
         for specs in reqs:
             handles.append(srv.submit(specs))
