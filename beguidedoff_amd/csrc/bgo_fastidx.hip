@@ -5,15 +5,20 @@
 // index here replaces the reference's FM-index: every forward position p has
 // a key, the 12 bases p..p+11 as 2-bit codes with base p in bits 22-23 (N read
 // as A, bases past the genome end as A).  Positions are grouped by key by a
-// counting sort on the GPU: count, exclusive scan, scatter.  An anchor of m
+// two-pass MSD radix placement on the GPU, 12 bits per pass, with per-block
+// histograms in LDS (no global atomics): pass 1 places (position, low digit)
+// by the key's high 12 bits; pass 2 places the positions of each high-digit
+// bucket by the low 12 bits.  An anchor of m
 // cold bases is then a contiguous key range (its first min(m, 12) bases as a
 // prefix), and the query checks each occurrence's window on the GPU: window
 // inside one contig, the whole cold block, the PAM, the 2-bit mismatches plus
 // the N positions that count as mismatches, and score <= cutoff.
 //
 // Memory: 4 bytes per position (12.4 GB at 3.09 Gbp) plus 64 MiB of bucket
-// offsets kept, and 2 x 64 MiB of counts and cursors during the build.  Runs
-// of one key (N runs, repeats) are counted and placed with one atomic per wave.
+// offsets kept; the build holds 6 more bytes per position (the pass-1
+// positions and low digits) and < 200 MiB of histograms and counts.  Runs of
+// one key in a wave (N runs, repeats) take one LDS atomic.
+#include <algorithm>
 #include <vector>
 
 #include "bgo_common.h"
@@ -71,34 +76,17 @@ __device__ inline uint32_t key_at(const uint2* __restrict__ planes, uint64_t p) 
 
 __device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-// Count keys; a wave whose active positions share its first lane's key adds
-// them with one atomic.
-__global__ void count_kernel(const uint2* __restrict__ planes, uint64_t n_pos,
-                             uint32_t* __restrict__ counts) {
-  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool ok = p < n_pos;
-  const uint32_t key = ok ? key_at(planes, p) : 0u;
-  const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
-  const bool same = ok && key == k0;
-  const uint64_t m = ballot64(same);
-  if (same) {
-    if (lanes_below(m) == 0) atomicAdd(&counts[k0], static_cast<uint32_t>(__popcll(m)));
-  } else if (ok) {
-    atomicAdd(&counts[key], 1u);
-  }
-}
-
-// Exclusive scan of kBuckets counts in three steps: per-chunk scan (chunk
-// totals out), scan of the chunk totals (one block), add the chunk bases.
-__global__ void scan_chunks_kernel(const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
-                                   uint32_t* __restrict__ totals) {
+// Exclusive scan of n u32 values in place (n a multiple of kScanChunk, the
+// array n + 1 long: data[n] receives the total) in three steps: per-chunk scan
+// (chunk totals out), scan of the chunk totals (one block), add the bases.
+__global__ void scan_chunks_kernel(uint32_t* __restrict__ data, uint32_t* __restrict__ totals) {
   __shared__ uint32_t part[256];
   const uint32_t t = threadIdx.x;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanChunk + t * 16;
   uint32_t v[16], s = 0;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    v[i] = counts[base + i];
+    v[i] = data[base + i];
     s += v[i];
   }
   part[t] = s;
@@ -112,23 +100,20 @@ __global__ void scan_chunks_kernel(const uint32_t* __restrict__ counts, uint32_t
   uint32_t run = part[t] - s;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    out[base + i] = run;
+    data[base + i] = run;
     run += v[i];
   }
   if (t == 255) totals[blockIdx.x] = part[255];
 }
 
+// One block of 256 threads; totals[n] receives the grand total.
 __global__ void scan_totals_kernel(uint32_t* __restrict__ totals, uint32_t n) {
-  // one block of 256 threads; n = kBuckets / kScanChunk = 4096 totals, 16 per thread
   __shared__ uint32_t part[256];
   const uint32_t t = threadIdx.x;
-  uint32_t v[16], s = 0;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t j = t * 16 + i;
-    v[i] = j < n ? totals[j] : 0u;
-    s += v[i];
-  }
+  const uint32_t per = (n + 255) / 256;
+  const uint32_t b = min(n, t * per), e = min(n, b + per);
+  uint32_t s = 0;
+  for (uint32_t j = b; j < e; ++j) s += totals[j];
   part[t] = s;
   __syncthreads();
   for (uint32_t d = 1; d < 256; d <<= 1) {
@@ -138,44 +123,167 @@ __global__ void scan_totals_kernel(uint32_t* __restrict__ totals, uint32_t n) {
     __syncthreads();
   }
   uint32_t run = part[t] - s;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t j = t * 16 + i;
-    if (j < n) totals[j] = run;
-    run += v[i];
+  for (uint32_t j = b; j < e; ++j) {
+    const uint32_t v = totals[j];
+    totals[j] = run;
+    run += v;
   }
+  if (t == 255) totals[n] = part[255];
 }
 
-__global__ void add_bases_kernel(uint32_t* __restrict__ out, const uint32_t* __restrict__ totals,
-                                 uint32_t* __restrict__ cursor, uint64_t n_pos) {
+__global__ void add_bases_kernel(uint32_t* __restrict__ data, const uint32_t* __restrict__ totals,
+                                 uint64_t n, uint32_t n_chunks) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < kBuckets) {
-    const uint32_t v = out[i] + totals[i / kScanChunk];
-    out[i] = v;
-    cursor[i] = v;
-  }
-  if (i == 0) out[kBuckets] = static_cast<uint32_t>(n_pos);
+  if (i < n) data[i] += totals[i / kScanChunk];
+  if (i == n) data[n] = totals[n_chunks];
 }
 
-// Place positions; runs of one key within a wave take one atomic.
-__global__ void scatter_kernel(const uint2* __restrict__ planes, uint64_t n_pos,
-                               uint32_t* __restrict__ cursor, uint32_t* __restrict__ pos_out) {
-  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool ok = p < n_pos;
-  const uint32_t key = ok ? key_at(planes, p) : 0u;
-  const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
-  const bool same = ok && key == k0;
+inline hipError_t exclusive_scan(uint32_t* data, uint64_t n, uint32_t* totals) {
+  const uint64_t chunks = n / kScanChunk;
+  hipLaunchKernelGGL(scan_chunks_kernel, dim3(static_cast<uint32_t>(chunks)), dim3(256), 0, 0,
+                     data, totals);
+  hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(256), 0, 0, totals,
+                     static_cast<uint32_t>(chunks));
+  hipLaunchKernelGGL(add_bases_kernel, dim3(static_cast<uint32_t>((n + 256) / 256)), dim3(256), 0,
+                     0, data, totals, n, static_cast<uint32_t>(chunks));
+  return hipGetLastError();
+}
+
+// Radix placement: 12-bit digits, kTile positions (or pass-1 entries) per block.
+constexpr uint32_t kDigitBins = 1u << 12;
+constexpr uint32_t kTile = 1u << 20;
+constexpr uint32_t kBuildThreads = 256;
+
+// LDS slot of this lane's element with digit d among the block's elements of
+// that digit (hist[d] counts them): lanes of a wave that share the first
+// lane's digit take their slots with one atomic, the others one each.
+__device__ inline uint32_t lds_slot(uint32_t* hist, uint32_t d, bool ok) {
+  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+  const bool same = ok && d == d0;
   const uint64_t m = ballot64(same);
   uint32_t base = 0;
-  if (same && lanes_below(m) == 0) base = atomicAdd(&cursor[k0], static_cast<uint32_t>(__popcll(m)));
-  // the leader's base to every lane of the run: the lowest set lane of m
+  if (same && lanes_below(m) == 0) base = atomicAdd(&hist[d0], static_cast<uint32_t>(__popcll(m)));
   const int leader = m ? static_cast<int>(__builtin_ctzll(m)) : 0;
   base = static_cast<uint32_t>(__shfl(static_cast<int>(base), leader, 64));
-  if (same) {
-    pos_out[base + lanes_below(m)] = static_cast<uint32_t>(p);
-  } else if (ok) {
-    pos_out[atomicAdd(&cursor[key], 1u)] = static_cast<uint32_t>(p);
+  if (same) return base + lanes_below(m);
+  if (ok) return atomicAdd(&hist[d], 1u);
+  return 0u;
+}
+
+// Pass 1 counts: H[hi * nb + b] = positions of block b whose key has high
+// digit hi.
+__global__ __launch_bounds__(kBuildThreads) void hi_hist_kernel(const uint2* __restrict__ planes,
+                                                                uint64_t n_pos, uint32_t nb,
+                                                                uint32_t* __restrict__ H) {
+  __shared__ uint32_t hist[kDigitBins];
+  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads) hist[i] = 0;
+  __syncthreads();
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kTile;
+  const uint64_t end = begin + kTile < n_pos ? begin + kTile : n_pos;
+  // every wave runs the same number of rounds (lds_slot is wave-wide)
+  for (uint64_t q = begin; q < end; q += kBuildThreads) {
+    const uint64_t p = q + threadIdx.x;
+    const bool ok = p < end;
+    (void)lds_slot(hist, ok ? key_at(planes, p) >> 12 : 0u, ok);
   }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads)
+    H[static_cast<uint64_t>(i) * nb + blockIdx.x] = hist[i];
+}
+
+// Pass 1 placement: O = exclusive scan of H (bin-major), so block b's
+// positions of high digit hi start at O[hi * nb + b].  Writes the position
+// and its key's low digit.
+__global__ __launch_bounds__(kBuildThreads) void hi_place_kernel(
+    const uint2* __restrict__ planes, uint64_t n_pos, uint32_t nb, const uint32_t* __restrict__ O,
+    uint32_t* __restrict__ pos1, uint16_t* __restrict__ lo1) {
+  __shared__ uint32_t next[kDigitBins];
+  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads)
+    next[i] = O[static_cast<uint64_t>(i) * nb + blockIdx.x];
+  __syncthreads();
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kTile;
+  const uint64_t end = begin + kTile < n_pos ? begin + kTile : n_pos;
+  for (uint64_t q = begin; q < end; q += kBuildThreads) {
+    const uint64_t p = q + threadIdx.x;
+    const bool ok = p < end;
+    const uint32_t key = ok ? key_at(planes, p) : 0u;
+    const uint32_t slot = lds_slot(next, key >> 12, ok);
+    if (ok) {
+      pos1[slot] = static_cast<uint32_t>(p);
+      lo1[slot] = static_cast<uint16_t>(key & 0xFFFu);
+    }
+  }
+}
+
+// Pass 2 work: block k places entries [begin[k], end[k]) of pass 1, all of
+// high digit hi[k].
+struct Pass2Block {
+  uint32_t begin, end, hi, pad;
+};
+
+// Pass 2 counts: H[lo * nb + k] = entries of block k with low digit lo.
+__global__ __launch_bounds__(kBuildThreads) void lo_hist_kernel(const uint16_t* __restrict__ lo1,
+                                                                const Pass2Block* __restrict__ blocks,
+                                                                uint32_t nb, uint32_t* __restrict__ H) {
+  __shared__ uint32_t hist[kDigitBins];
+  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads) hist[i] = 0;
+  __syncthreads();
+  const Pass2Block w = blocks[blockIdx.x];
+  // 64-bit cursor: w.end may lie within one step of 2^32
+  for (uint64_t q = w.begin; q < w.end; q += kBuildThreads) {
+    const uint64_t i = q + threadIdx.x;
+    const bool ok = i < w.end;
+    (void)lds_slot(hist, ok ? lo1[i] : 0u, ok);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads)
+    H[static_cast<uint64_t>(i) * nb + blockIdx.x] = hist[i];
+}
+
+// Entries of key hi:lo: S is the exclusive scan of the pass-2 counts (bin-
+// major, S[4096 nb] = total), and the blocks of high digit hi are
+// first[hi] .. first[hi + 1] - 1, so the key's count is a difference of S
+// along row lo.
+__global__ void key_count_kernel(const uint32_t* __restrict__ S, const uint32_t* __restrict__ first,
+                                 uint32_t nb, uint32_t* __restrict__ counts) {
+  const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= kBuckets) return;
+  const uint32_t hi = key >> 12, lo = key & 0xFFFu;
+  const uint64_t row = static_cast<uint64_t>(lo) * nb;
+  counts[key] = S[row + first[hi + 1]] - S[row + first[hi]];
+}
+
+// Pass 2 placement: block k's entries of low digit lo start at the key's
+// bucket offset plus the entries of that key in the earlier blocks of its
+// high digit.
+__global__ __launch_bounds__(kBuildThreads) void lo_place_kernel(
+    const uint32_t* __restrict__ pos1, const uint16_t* __restrict__ lo1,
+    const Pass2Block* __restrict__ blocks, const uint32_t* __restrict__ first, uint32_t nb,
+    const uint32_t* __restrict__ S, const uint32_t* __restrict__ offsets,
+    uint32_t* __restrict__ pos_out) {
+  __shared__ uint32_t next[kDigitBins];
+  const Pass2Block w = blocks[blockIdx.x];
+  const uint32_t f = first[w.hi];
+  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads) {
+    const uint64_t row = static_cast<uint64_t>(i) * nb;
+    next[i] = offsets[(w.hi << 12) | i] + (S[row + blockIdx.x] - S[row + f]);
+  }
+  __syncthreads();
+  for (uint64_t q = w.begin; q < w.end; q += kBuildThreads) {
+    const uint64_t i = q + threadIdx.x;
+    const bool ok = i < w.end;
+    const uint32_t slot = lds_slot(next, ok ? lo1[i] : 0u, ok);
+    if (ok) pos_out[slot] = pos1[i];
+  }
+}
+
+// First entry of each high digit after pass 1 (O[hi * nb], the scan of the
+// pass-1 counts); starts[kDigitBins] = n_pos.
+__global__ void coarse_starts_kernel(const uint32_t* __restrict__ O, uint32_t nb, uint64_t n_pos,
+                                     uint32_t* __restrict__ starts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kDigitBins) starts[i] = O[static_cast<uint64_t>(i) * nb];
+  if (i == kDigitBins) starts[i] = static_cast<uint32_t>(n_pos);
 }
 
 // Occurrence chunks of each anchor's key range.
@@ -335,6 +443,82 @@ void free_query_buffers(bgo_fast_index* X) {
 
 using namespace bgo;
 
+namespace bgo {
+namespace fi {
+
+// Positions 0 .. n - 1 grouped by key into pos_out, bucket starts into
+// offsets[0 .. kBuckets] (the two radix passes above); the DevBufs hold the
+// temporaries.
+hipError_t build_passes(const uint2* planes, uint64_t n, uint32_t* offsets, uint32_t* pos_out,
+                        DevBuf& h1, DevBuf& totals, DevBuf& pos1, DevBuf& lo1, DevBuf& starts,
+                        DevBuf& blocks_d, DevBuf& first_d, DevBuf& h2) {
+  const uint32_t nb1 = static_cast<uint32_t>((n + kTile - 1) / kTile);
+  const uint64_t max_chunks = std::max<uint64_t>(kBuckets / kScanChunk, kDigitBins * 2u);
+  hipError_t e = hipMalloc(&h1.p, (static_cast<size_t>(kDigitBins) * nb1 + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&pos1.p, n * 4);
+  if (e == hipSuccess) e = hipMalloc(&lo1.p, n * 2);
+  if (e == hipSuccess) e = hipMalloc(&starts.p, (kDigitBins + 1) * 4);
+  // chunk totals of the three scans: nb1, nb2 (<= nb1 + kDigitBins) and 4096 chunks
+  if (e == hipSuccess) e = hipMalloc(&totals.p, (std::max<uint64_t>(max_chunks, 2ull * nb1 + kDigitBins) + 1) * 4);
+  if (e != hipSuccess) return e;
+  auto* H1 = static_cast<uint32_t*>(h1.p);
+  auto* P1 = static_cast<uint32_t*>(pos1.p);
+  auto* L1 = static_cast<uint16_t*>(lo1.p);
+  auto* tot = static_cast<uint32_t*>(totals.p);
+  // pass 1: by the high 12 bits
+  hipLaunchKernelGGL(hi_hist_kernel, dim3(nb1), dim3(kBuildThreads), 0, 0, planes, n, nb1, H1);
+  if ((e = exclusive_scan(H1, static_cast<uint64_t>(kDigitBins) * nb1, tot)) != hipSuccess) return e;
+  hipLaunchKernelGGL(hi_place_kernel, dim3(nb1), dim3(kBuildThreads), 0, 0, planes, n, nb1, H1,
+                     P1, L1);
+  hipLaunchKernelGGL(coarse_starts_kernel, dim3((kDigitBins + 256) / 256), dim3(256), 0, 0, H1,
+                     nb1, n, static_cast<uint32_t*>(starts.p));
+  std::vector<uint32_t> st(kDigitBins + 1);
+  if ((e = hipMemcpy(st.data(), starts.p, st.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+    return e;
+  // pass 2 work: each high digit's entries in blocks of at most kTile
+  std::vector<Pass2Block> blocks;
+  std::vector<uint32_t> first(kDigitBins + 1);
+  for (uint32_t hi = 0; hi < kDigitBins; ++hi) {
+    first[hi] = static_cast<uint32_t>(blocks.size());
+    for (uint64_t b = st[hi]; b < st[hi + 1]; b += kTile)
+      blocks.push_back({static_cast<uint32_t>(b),
+                        static_cast<uint32_t>(std::min<uint64_t>(b + kTile, st[hi + 1])), hi, 0});
+  }
+  first[kDigitBins] = static_cast<uint32_t>(blocks.size());
+  const uint32_t nb2 = static_cast<uint32_t>(blocks.size());
+  if (nb2 == 0 || nb2 > 2ull * nb1 + kDigitBins) return hipErrorInvalidValue;
+  e = hipMalloc(&blocks_d.p, blocks.size() * sizeof(Pass2Block));
+  if (e == hipSuccess) e = hipMalloc(&first_d.p, first.size() * 4);
+  if (e == hipSuccess) e = hipMalloc(&h2.p, (static_cast<size_t>(kDigitBins) * nb2 + 1) * 4);
+  if (e == hipSuccess)
+    e = hipMemcpy(blocks_d.p, blocks.data(), blocks.size() * sizeof(Pass2Block), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(first_d.p, first.data(), first.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  auto* B2 = static_cast<const Pass2Block*>(blocks_d.p);
+  auto* F2 = static_cast<const uint32_t*>(first_d.p);
+  auto* H2 = static_cast<uint32_t*>(h2.p);
+  // pass 2: by the low 12 bits within each high digit
+  hipLaunchKernelGGL(lo_hist_kernel, dim3(nb2), dim3(kBuildThreads), 0, 0, L1, B2, nb2, H2);
+  if ((e = exclusive_scan(H2, static_cast<uint64_t>(kDigitBins) * nb2, tot)) != hipSuccess) return e;
+  hipLaunchKernelGGL(key_count_kernel, dim3(kBuckets / 256), dim3(256), 0, 0, H2, F2, nb2, offsets);
+  if ((e = exclusive_scan(offsets, kBuckets, tot)) != hipSuccess) return e;  // offsets[kBuckets] = n
+  hipLaunchKernelGGL(lo_place_kernel, dim3(nb2), dim3(kBuildThreads), 0, 0, P1, L1, B2, F2, nb2,
+                     H2, offsets, pos_out);
+  return hipGetLastError();
+}
+
+hipError_t build_index(const uint2* planes, uint64_t n, uint32_t* offsets, uint32_t* pos_out) {
+  DevBuf h1, totals, pos1, lo1, starts, blocks_d, first_d, h2;
+  const hipError_t e = build_passes(planes, n, offsets, pos_out, h1, totals, pos1, lo1, starts,
+                                    blocks_d, first_d, h2);
+  // no kernel may still use the temporaries when they are freed
+  const hipError_t s = hipDeviceSynchronize();
+  return e != hipSuccess ? e : s;
+}
+
+}  // namespace fi
+}  // namespace bgo
+
 extern "C" int bgo_fast_index_create(bgo_genome* g, const int64_t* contig_offsets,
                                      const int64_t* contig_lengths, int32_t n_contigs,
                                      bgo_fast_index** out) {
@@ -365,7 +549,6 @@ extern "C" int bgo_fast_index_create(bgo_genome* g, const int64_t* contig_offset
     delete X;
     return rc;
   };
-  fi::DevBuf counts, cursor, totals;
   hipError_t e = hipMalloc(&X->d_offsets, (fi::kBuckets + 1) * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&X->d_pos, X->n_pos * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&X->d_contigs, 2 * static_cast<size_t>(n_contigs) * 8);
@@ -373,34 +556,13 @@ extern "C" int bgo_fast_index_create(bgo_genome* g, const int64_t* contig_offset
   if (e == hipSuccess) e = hipMalloc(&X->d_count, sizeof(unsigned long long));
   if (e == hipSuccess) e = hipEventCreate(&X->ev[0]);
   if (e == hipSuccess) e = hipEventCreate(&X->ev[1]);
-  if (e == hipSuccess) e = hipMalloc(&counts.p, fi::kBuckets * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&cursor.p, fi::kBuckets * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&totals.p, (fi::kBuckets / fi::kScanChunk) * sizeof(uint32_t));
-  if (e != hipSuccess)
-    return cleanup(fail(BGO_ERR_HIP, "fast index allocation failed: %s", hipGetErrorString(e)));
-  e = hipMemcpy(X->d_contigs, contig_offsets, n_contigs * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(X->d_contigs, contig_offsets, n_contigs * 8, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(X->d_contigs + n_contigs, contig_lengths, n_contigs * 8, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(counts.p, 0, fi::kBuckets * sizeof(uint32_t));
-  const uint2* planes = g->d_planes[0];
-  const uint64_t pos_blocks = (X->n_pos + 255) / 256;
-  auto* c = static_cast<uint32_t*>(counts.p);
-  auto* cur = static_cast<uint32_t*>(cursor.p);
-  auto* tot = static_cast<uint32_t*>(totals.p);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(fi::count_kernel, dim3(static_cast<uint32_t>(pos_blocks)), dim3(256), 0, 0,
-                       planes, X->n_pos, c);
-    hipLaunchKernelGGL(fi::scan_chunks_kernel, dim3(fi::kBuckets / fi::kScanChunk), dim3(256), 0, 0,
-                       c, X->d_offsets, tot);
-    hipLaunchKernelGGL(fi::scan_totals_kernel, dim3(1), dim3(256), 0, 0, tot,
-                       fi::kBuckets / fi::kScanChunk);
-    hipLaunchKernelGGL(fi::add_bases_kernel, dim3(fi::kBuckets / 256), dim3(256), 0, 0,
-                       X->d_offsets, tot, cur, X->n_pos);
-    hipLaunchKernelGGL(fi::scatter_kernel, dim3(static_cast<uint32_t>(pos_blocks)), dim3(256), 0,
-                       0, planes, X->n_pos, cur, X->d_pos);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess)
+    return cleanup(fail(BGO_ERR_HIP, "fast index allocation failed: %s", hipGetErrorString(e)));
+  e = fi::build_index(g->d_planes[0], X->n_pos, X->d_offsets, X->d_pos);
   if (e != hipSuccess)
     return cleanup(fail(BGO_ERR_HIP, "fast index build failed: %s", hipGetErrorString(e)));
   X->bytes = static_cast<int64_t>((fi::kBuckets + 1) * 4 + X->n_pos * 4 + 16 * n_contigs);

@@ -284,6 +284,48 @@ def test_index_on_repeats_and_n_runs(gpu, cutoff):
 
 
 @pytest.mark.gpu
+def test_index_contents_equal_a_cpu_counting_sort(gpu):
+    """The bucket table equals a CPU bincount of the 12-mer keys (base p in
+    the top bits, N and bases past the end as A) and the position array is a
+    permutation of all positions grouped in key order.  The genome spans four
+    build tiles of 2^20 positions, and its all-A key (a 1.3 Mbp N run) fills
+    two tiles of its high-digit bucket."""
+    from beguidedoff_amd.batched import DeviceGenome
+    rng = np.random.default_rng(11)
+
+    def rnd(n):
+        return "".join(rng.choice(list("ACGT"), size=n))
+    contigs = [("chr1", rnd(1_200_000) + "N" * 1_300_000 + rnd(400_000)),
+               ("chr2", "T" * 50_000 + rnd(900_001)),
+               ("chr3", rnd(777))]
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    lens = np.array([len(s) for _, s in contigs], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    idx = FastIndexGPU(dg, offs, lens)
+    try:
+        lut = np.zeros(256, dtype=np.uint32)
+        for b, c in CODE.items():
+            lut[ord(b)] = c
+        codes = lut[np.frombuffer("".join(s for _, s in contigs).encode(), dtype=np.uint8)]
+        n = codes.size
+        padded = np.concatenate([codes, np.zeros(11, dtype=np.uint32)])
+        keys = np.zeros(n, dtype=np.uint32)
+        for i in range(12):
+            keys |= padded[i:i + n] << np.uint32(22 - 2 * i)
+        want = np.zeros((1 << 24) + 1, dtype=np.int64)
+        want[1:] = np.cumsum(np.bincount(keys.astype(np.int64), minlength=1 << 24))
+        assert want[4096] > 1 << 20  # high digit 0 (the N run): two pass-2 tiles
+        assert np.array_equal(idx.offsets().astype(np.int64), want)
+        pos = idx.positions(0, n)
+        assert np.array_equal(np.sort(pos), np.arange(n, dtype=np.uint32))
+        assert np.all(np.diff(keys[pos].astype(np.int64)) >= 0)
+    finally:
+        idx.close()
+        dg.close()
+
+
+@pytest.mark.gpu
 def test_index_query_grows_its_buffer_and_sizes(gpu):
     contigs, guides = repeat_case(4)
     genome = synth.genome_from_strings(contigs)
