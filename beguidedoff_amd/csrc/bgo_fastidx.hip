@@ -5,19 +5,18 @@
 // index here replaces the reference's FM-index: every forward position p has
 // a key, the 12 bases p..p+11 as 2-bit codes with base p in bits 22-23 (N read
 // as A, bases past the genome end as A).  Positions are grouped by key by a
-// two-pass MSD radix placement on the GPU, 12 bits per pass, with per-block
-// histograms in LDS (no global atomics): pass 1 places (position, low digit)
-// by the key's high 12 bits; pass 2 places the positions of each high-digit
-// bucket by the low 12 bits.  An anchor of m
+// three-pass MSD radix placement on the GPU, 8 bits per pass, with per-block
+// histograms in LDS (no global atomics) and each block round sorted by digit
+// in LDS before it is written, so stores leave as runs.  An anchor of m
 // cold bases is then a contiguous key range (its first min(m, 12) bases as a
 // prefix), and the query checks each occurrence's window on the GPU: window
 // inside one contig, the whole cold block, the PAM, the 2-bit mismatches plus
 // the N positions that count as mismatches, and score <= cutoff.
 //
 // Memory: 4 bytes per position (12.4 GB at 3.09 Gbp) plus 64 MiB of bucket
-// offsets kept; the build holds 6 more bytes per position (the pass-1
-// positions and low digits) and < 200 MiB of histograms and counts.  Runs of
-// one key in a wave (N runs, repeats) take one LDS atomic.
+// offsets kept; the build holds 7 more bytes per position (positions and
+// key bits between passes) and < 100 MiB of histograms and block tables.
+// Runs of one key in a wave (N runs, repeats) take one LDS atomic.
 #include <algorithm>
 #include <vector>
 
@@ -149,14 +148,24 @@ inline hipError_t exclusive_scan(uint32_t* data, uint64_t n, uint32_t* totals) {
   return hipGetLastError();
 }
 
-// Radix placement: 12-bit digits, kTile positions (or pass-1 entries) per block.
-constexpr uint32_t kDigitBins = 1u << 12;
+// Radix placement: three MSD passes of 8 bits of the key (bits 16-23, 8-15,
+// 0-7).  A pass works on segments (entries of equal higher digits; pass 1:
+// all positions), each cut into blocks of at most kTile entries.  Pass 1
+// writes each position and the key's low 16 bits, pass 2 the position and the
+// low 8 bits, pass 3 the position alone.
+constexpr uint32_t kDigitBins = 256;
 constexpr uint32_t kTile = 1u << 20;
 constexpr uint32_t kBuildThreads = 256;
+constexpr uint32_t kRound = 4096;  // entries staged in LDS per block round
+constexpr uint32_t kPerThread = kRound / kBuildThreads;
+
+struct PassBlock {
+  uint32_t begin, end, seg, pad;
+};
 
 // LDS slot of this lane's element with digit d among the block's elements of
-// that digit (hist[d] counts them): lanes of a wave that share the first
-// lane's digit take their slots with one atomic, the others one each.
+// that digit so far (hist[d] counts them): lanes of a wave that share the
+// first lane's digit take their slots with one atomic, the others one each.
 __device__ inline uint32_t lds_slot(uint32_t* hist, uint32_t d, bool ok) {
   const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
   const bool same = ok && d == d0;
@@ -170,120 +179,157 @@ __device__ inline uint32_t lds_slot(uint32_t* hist, uint32_t d, bool ok) {
   return 0u;
 }
 
-// Pass 1 counts: H[hi * nb + b] = positions of block b whose key has high
-// digit hi.
-__global__ __launch_bounds__(kBuildThreads) void hi_hist_kernel(const uint2* __restrict__ planes,
-                                                                uint64_t n_pos, uint32_t nb,
-                                                                uint32_t* __restrict__ H) {
-  __shared__ uint32_t hist[kDigitBins];
-  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads) hist[i] = 0;
-  __syncthreads();
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kTile;
-  const uint64_t end = begin + kTile < n_pos ? begin + kTile : n_pos;
-  // every wave runs the same number of rounds (lds_slot is wave-wide)
-  for (uint64_t q = begin; q < end; q += kBuildThreads) {
-    const uint64_t p = q + threadIdx.x;
-    const bool ok = p < end;
-    (void)lds_slot(hist, ok ? key_at(planes, p) >> 12 : 0u, ok);
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads)
-    H[static_cast<uint64_t>(i) * nb + blockIdx.x] = hist[i];
-}
-
-// Pass 1 placement: O = exclusive scan of H (bin-major), so block b's
-// positions of high digit hi start at O[hi * nb + b].  Writes the position
-// and its key's low digit.
-__global__ __launch_bounds__(kBuildThreads) void hi_place_kernel(
-    const uint2* __restrict__ planes, uint64_t n_pos, uint32_t nb, const uint32_t* __restrict__ O,
-    uint32_t* __restrict__ pos1, uint16_t* __restrict__ lo1) {
-  __shared__ uint32_t next[kDigitBins];
-  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads)
-    next[i] = O[static_cast<uint64_t>(i) * nb + blockIdx.x];
-  __syncthreads();
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kTile;
-  const uint64_t end = begin + kTile < n_pos ? begin + kTile : n_pos;
-  for (uint64_t q = begin; q < end; q += kBuildThreads) {
-    const uint64_t p = q + threadIdx.x;
-    const bool ok = p < end;
-    const uint32_t key = ok ? key_at(planes, p) : 0u;
-    const uint32_t slot = lds_slot(next, key >> 12, ok);
-    if (ok) {
-      pos1[slot] = static_cast<uint32_t>(p);
-      lo1[slot] = static_cast<uint16_t>(key & 0xFFFu);
-    }
+// Element i of pass P: its position, digit and the key bits below the digit.
+template <int P>
+__device__ inline void pass_elem(uint64_t i, const uint2* __restrict__ planes,
+                                 const uint32_t* __restrict__ pos_in,
+                                 const uint16_t* __restrict__ r16_in,
+                                 const uint8_t* __restrict__ r8_in, uint32_t& pos, uint32_t& digit,
+                                 uint32_t& rest) {
+  if constexpr (P == 1) {
+    const uint32_t key = key_at(planes, i);
+    pos = static_cast<uint32_t>(i);
+    digit = key >> 16;
+    rest = key & 0xFFFFu;
+  } else if constexpr (P == 2) {
+    const uint32_t v = r16_in[i];
+    pos = pos_in[i];
+    digit = v >> 8;
+    rest = v & 0xFFu;
+  } else {
+    pos = pos_in[i];
+    digit = r8_in[i];
+    rest = 0;
   }
 }
 
-// Pass 2 work: block k places entries [begin[k], end[k]) of pass 1, all of
-// high digit hi[k].
-struct Pass2Block {
-  uint32_t begin, end, hi, pad;
+template <int P>
+__device__ inline uint32_t pass_digit(uint64_t i, const uint2* __restrict__ planes,
+                                      const uint16_t* __restrict__ r16_in,
+                                      const uint8_t* __restrict__ r8_in) {
+  if constexpr (P == 1) return key_at(planes, i) >> 16;
+  else if constexpr (P == 2) return static_cast<uint32_t>(r16_in[i]) >> 8;
+  else return r8_in[i];
+}
+
+struct PassArgs {
+  const uint2* planes;
+  const uint32_t* pos_in;
+  const uint16_t* r16_in;
+  const uint8_t* r8_in;
+  uint32_t* pos_out;
+  uint16_t* r16_out;
+  uint8_t* r8_out;
+  const PassBlock* blocks;
+  const uint32_t* first;      // [segments + 1] first block of each segment
+  const uint32_t* seg_start;  // [segments + 1] first entry of each segment
+  const uint32_t* S;          // exclusive scan of the counts H (digit-major, stride nbp)
+  uint32_t nbp;               // block columns of H (a multiple of 16)
 };
 
-// Pass 2 counts: H[lo * nb + k] = entries of block k with low digit lo.
-__global__ __launch_bounds__(kBuildThreads) void lo_hist_kernel(const uint16_t* __restrict__ lo1,
-                                                                const Pass2Block* __restrict__ blocks,
-                                                                uint32_t nb, uint32_t* __restrict__ H) {
+// Counts: H[d * nbp + k] = entries of block k with digit d.
+template <int P>
+__global__ __launch_bounds__(kBuildThreads) void pass_hist_kernel(PassArgs a, uint32_t* __restrict__ H) {
   __shared__ uint32_t hist[kDigitBins];
-  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads) hist[i] = 0;
+  hist[threadIdx.x] = 0;
   __syncthreads();
-  const Pass2Block w = blocks[blockIdx.x];
-  // 64-bit cursor: w.end may lie within one step of 2^32
+  const PassBlock w = a.blocks[blockIdx.x];
+  // every wave runs the same rounds (lds_slot is wave-wide)
   for (uint64_t q = w.begin; q < w.end; q += kBuildThreads) {
     const uint64_t i = q + threadIdx.x;
     const bool ok = i < w.end;
-    (void)lds_slot(hist, ok ? lo1[i] : 0u, ok);
+    (void)lds_slot(hist, ok ? pass_digit<P>(i, a.planes, a.r16_in, a.r8_in) : 0u, ok);
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads)
-    H[static_cast<uint64_t>(i) * nb + blockIdx.x] = hist[i];
+  H[static_cast<uint64_t>(threadIdx.x) * a.nbp + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Entries of key hi:lo: S is the exclusive scan of the pass-2 counts (bin-
-// major, S[4096 nb] = total), and the blocks of high digit hi are
-// first[hi] .. first[hi + 1] - 1, so the key's count is a difference of S
-// along row lo.
-__global__ void key_count_kernel(const uint32_t* __restrict__ S, const uint32_t* __restrict__ first,
-                                 uint32_t nb, uint32_t* __restrict__ counts) {
-  const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
-  if (key >= kBuckets) return;
-  const uint32_t hi = key >> 12, lo = key & 0xFFFu;
-  const uint64_t row = static_cast<uint64_t>(lo) * nb;
-  counts[key] = S[row + first[hi + 1]] - S[row + first[hi]];
-}
-
-// Pass 2 placement: block k's entries of low digit lo start at the key's
-// bucket offset plus the entries of that key in the earlier blocks of its
-// high digit.
-__global__ __launch_bounds__(kBuildThreads) void lo_place_kernel(
-    const uint32_t* __restrict__ pos1, const uint16_t* __restrict__ lo1,
-    const Pass2Block* __restrict__ blocks, const uint32_t* __restrict__ first, uint32_t nb,
-    const uint32_t* __restrict__ S, const uint32_t* __restrict__ offsets,
-    uint32_t* __restrict__ pos_out) {
-  __shared__ uint32_t next[kDigitBins];
-  const Pass2Block w = blocks[blockIdx.x];
-  const uint32_t f = first[w.hi];
-  for (uint32_t i = threadIdx.x; i < kDigitBins; i += kBuildThreads) {
-    const uint64_t row = static_cast<uint64_t>(i) * nb;
-    next[i] = offsets[(w.hi << 12) | i] + (S[row + blockIdx.x] - S[row + f]);
-  }
+// Exclusive scan of v over the block's 256 threads (x: LDS scratch of 256).
+__device__ inline uint32_t block_scan256(uint32_t* x, uint32_t v) {
+  const uint32_t t = threadIdx.x;
+  x[t] = v;
   __syncthreads();
-  for (uint64_t q = w.begin; q < w.end; q += kBuildThreads) {
-    const uint64_t i = q + threadIdx.x;
-    const bool ok = i < w.end;
-    const uint32_t slot = lds_slot(next, ok ? lo1[i] : 0u, ok);
-    if (ok) pos_out[slot] = pos1[i];
+  for (uint32_t d = 1; d < 256; d <<= 1) {
+    const uint32_t y = t >= d ? x[t - d] : 0u;
+    __syncthreads();
+    x[t] += y;
+    __syncthreads();
+  }
+  const uint32_t r = x[t] - v;
+  __syncthreads();
+  return r;
+}
+
+// Placement: block k of segment s writes its entries of digit d from
+// seg_start[s] + (entries of s with a smaller digit) + (entries of digit d in
+// the earlier blocks of s).  Each round of kRound entries is sorted by digit
+// in LDS first, so the stores go out as runs per digit.
+template <int P>
+__global__ __launch_bounds__(kBuildThreads) void pass_place_kernel(PassArgs a) {
+  __shared__ uint32_t gbase[kDigitBins], cnt[kDigitBins], lstart[kDigitBins], tmp[kDigitBins];
+  __shared__ uint32_t st_pos[kRound];
+  __shared__ uint16_t st_rest[kRound];
+  __shared__ uint8_t st_dig[kRound];
+  const uint32_t t = threadIdx.x;
+  const PassBlock w = a.blocks[blockIdx.x];
+  {
+    const uint64_t row = static_cast<uint64_t>(t) * a.nbp;
+    const uint32_t f = a.first[w.seg], f1 = a.first[w.seg + 1];
+    const uint32_t in_seg = a.S[row + f1] - a.S[row + f];
+    const uint32_t before = a.S[row + blockIdx.x] - a.S[row + f];
+    gbase[t] = a.seg_start[w.seg] + block_scan256(tmp, in_seg) + before;
+  }
+  for (uint64_t q = w.begin; q < w.end; q += kRound) {
+    cnt[t] = 0;
+    __syncthreads();
+    uint32_t pos[kPerThread], dig[kPerThread], rest[kPerThread], rank[kPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kPerThread; ++j) {
+      const uint64_t i = q + j * kBuildThreads + t;
+      const bool ok = i < w.end;
+      pos[j] = dig[j] = rest[j] = 0;
+      if (ok) pass_elem<P>(i, a.planes, a.pos_in, a.r16_in, a.r8_in, pos[j], dig[j], rest[j]);
+      rank[j] = lds_slot(cnt, dig[j], ok);
+    }
+    __syncthreads();
+    lstart[t] = block_scan256(tmp, cnt[t]);
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kPerThread; ++j) {
+      const uint64_t i = q + j * kBuildThreads + t;
+      if (i < w.end) {
+        const uint32_t slot = lstart[dig[j]] + rank[j];
+        st_pos[slot] = pos[j];
+        st_rest[slot] = static_cast<uint16_t>(rest[j]);
+        st_dig[slot] = static_cast<uint8_t>(dig[j]);
+      }
+    }
+    __syncthreads();
+    const uint32_t m = static_cast<uint32_t>(w.end - q < kRound ? w.end - q : kRound);
+    for (uint32_t e = t; e < m; e += kBuildThreads) {
+      const uint32_t d = st_dig[e];
+      const uint32_t out = gbase[d] + (e - lstart[d]);
+      a.pos_out[out] = st_pos[e];
+      if constexpr (P == 1) a.r16_out[out] = st_rest[e];
+      if constexpr (P == 2) a.r8_out[out] = static_cast<uint8_t>(st_rest[e]);
+    }
+    __syncthreads();
+    gbase[t] += cnt[t];
+    __syncthreads();
   }
 }
 
-// First entry of each high digit after pass 1 (O[hi * nb], the scan of the
-// pass-1 counts); starts[kDigitBins] = n_pos.
-__global__ void coarse_starts_kernel(const uint32_t* __restrict__ O, uint32_t nb, uint64_t n_pos,
-                                     uint32_t* __restrict__ starts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kDigitBins) starts[i] = O[static_cast<uint64_t>(i) * nb];
-  if (i == kDigitBins) starts[i] = static_cast<uint32_t>(n_pos);
+// First entry of each (segment, digit) after a pass: out[s * 256 + d], and
+// out[segments * 256] = n.  One block per segment.
+__global__ __launch_bounds__(kBuildThreads) void seg_starts_kernel(PassArgs a, uint64_t n,
+                                                                   uint32_t n_seg,
+                                                                   uint32_t* __restrict__ out) {
+  __shared__ uint32_t tmp[kDigitBins];
+  const uint32_t s = blockIdx.x, t = threadIdx.x;
+  const uint64_t row = static_cast<uint64_t>(t) * a.nbp;
+  const uint32_t in_seg = a.S[row + a.first[s + 1]] - a.S[row + a.first[s]];
+  out[static_cast<uint64_t>(s) * kDigitBins + t] = a.seg_start[s] + block_scan256(tmp, in_seg);
+  if (s == 0 && t == 0) out[static_cast<uint64_t>(n_seg) * kDigitBins] = static_cast<uint32_t>(n);
 }
 
 // Occurrence chunks of each anchor's key range.
@@ -447,70 +493,90 @@ namespace bgo {
 namespace fi {
 
 // Positions 0 .. n - 1 grouped by key into pos_out, bucket starts into
-// offsets[0 .. kBuckets] (the two radix passes above); the DevBufs hold the
+// offsets[0 .. kBuckets] (the three radix passes above); the DevBufs hold the
 // temporaries.
 hipError_t build_passes(const uint2* planes, uint64_t n, uint32_t* offsets, uint32_t* pos_out,
-                        DevBuf& h1, DevBuf& totals, DevBuf& pos1, DevBuf& lo1, DevBuf& starts,
-                        DevBuf& blocks_d, DevBuf& first_d, DevBuf& h2) {
-  const uint32_t nb1 = static_cast<uint32_t>((n + kTile - 1) / kTile);
-  const uint64_t max_chunks = std::max<uint64_t>(kBuckets / kScanChunk, kDigitBins * 2u);
-  hipError_t e = hipMalloc(&h1.p, (static_cast<size_t>(kDigitBins) * nb1 + 1) * 4);
-  if (e == hipSuccess) e = hipMalloc(&pos1.p, n * 4);
-  if (e == hipSuccess) e = hipMalloc(&lo1.p, n * 2);
-  if (e == hipSuccess) e = hipMalloc(&starts.p, (kDigitBins + 1) * 4);
-  // chunk totals of the three scans: nb1, nb2 (<= nb1 + kDigitBins) and 4096 chunks
-  if (e == hipSuccess) e = hipMalloc(&totals.p, (std::max<uint64_t>(max_chunks, 2ull * nb1 + kDigitBins) + 1) * 4);
+                        DevBuf& h, DevBuf& totals, DevBuf& pos_tmp, DevBuf& r16, DevBuf& r8,
+                        DevBuf& blocks_d, DevBuf& first_d, DevBuf& starts_a, DevBuf& starts_b) {
+  // blocks of a pass: at most n / kTile + one per segment (65536 in pass 3)
+  const uint64_t nb_max = n / kTile + 1 + (1u << 16);
+  const uint64_t nbp_max = (nb_max + 15) / 16 * 16;
+  hipError_t e = hipMalloc(&h.p, (kDigitBins * nbp_max + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&totals.p, (nbp_max / 16 + 2) * 4);
+  if (e == hipSuccess) e = hipMalloc(&pos_tmp.p, n * 4);
+  if (e == hipSuccess) e = hipMalloc(&r16.p, n * 2);
+  if (e == hipSuccess) e = hipMalloc(&r8.p, n);
+  if (e == hipSuccess) e = hipMalloc(&blocks_d.p, nb_max * sizeof(PassBlock));
+  if (e == hipSuccess) e = hipMalloc(&first_d.p, ((1u << 16) + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&starts_a.p, (kDigitBins + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&starts_b.p, ((1u << 16) + 1) * 4);
   if (e != hipSuccess) return e;
-  auto* H1 = static_cast<uint32_t*>(h1.p);
-  auto* P1 = static_cast<uint32_t*>(pos1.p);
-  auto* L1 = static_cast<uint16_t*>(lo1.p);
+  auto* H = static_cast<uint32_t*>(h.p);
   auto* tot = static_cast<uint32_t*>(totals.p);
-  // pass 1: by the high 12 bits
-  hipLaunchKernelGGL(hi_hist_kernel, dim3(nb1), dim3(kBuildThreads), 0, 0, planes, n, nb1, H1);
-  if ((e = exclusive_scan(H1, static_cast<uint64_t>(kDigitBins) * nb1, tot)) != hipSuccess) return e;
-  hipLaunchKernelGGL(hi_place_kernel, dim3(nb1), dim3(kBuildThreads), 0, 0, planes, n, nb1, H1,
-                     P1, L1);
-  hipLaunchKernelGGL(coarse_starts_kernel, dim3((kDigitBins + 256) / 256), dim3(256), 0, 0, H1,
-                     nb1, n, static_cast<uint32_t*>(starts.p));
-  std::vector<uint32_t> st(kDigitBins + 1);
-  if ((e = hipMemcpy(st.data(), starts.p, st.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
-    return e;
-  // pass 2 work: each high digit's entries in blocks of at most kTile
-  std::vector<Pass2Block> blocks;
-  std::vector<uint32_t> first(kDigitBins + 1);
-  for (uint32_t hi = 0; hi < kDigitBins; ++hi) {
-    first[hi] = static_cast<uint32_t>(blocks.size());
-    for (uint64_t b = st[hi]; b < st[hi + 1]; b += kTile)
-      blocks.push_back({static_cast<uint32_t>(b),
-                        static_cast<uint32_t>(std::min<uint64_t>(b + kTile, st[hi + 1])), hi, 0});
+  // pass outputs: positions to pos_out, then pos_tmp, then pos_out
+  uint32_t* pos_buf[3] = {pos_out, static_cast<uint32_t*>(pos_tmp.p), pos_out};
+  // segment starts of the current pass, host and device
+  std::vector<uint32_t> st = {0u, static_cast<uint32_t>(n)};
+  const uint32_t zero_n[2] = {0u, static_cast<uint32_t>(n)};
+  uint32_t* seg_d = static_cast<uint32_t*>(starts_b.p);
+  if ((e = hipMemcpy(seg_d, zero_n, sizeof(zero_n), hipMemcpyHostToDevice)) != hipSuccess) return e;
+  std::vector<PassBlock> blocks;
+  std::vector<uint32_t> first;
+  uint32_t n_seg = 1;
+  for (int pass = 1; pass <= 3; ++pass) {
+    blocks.clear();
+    first.assign(n_seg + 1, 0);
+    for (uint32_t sg = 0; sg < n_seg; ++sg) {
+      first[sg] = static_cast<uint32_t>(blocks.size());
+      for (uint64_t b = st[sg]; b < st[sg + 1]; b += kTile)
+        blocks.push_back({static_cast<uint32_t>(b),
+                          static_cast<uint32_t>(std::min<uint64_t>(b + kTile, st[sg + 1])), sg, 0});
+    }
+    first[n_seg] = static_cast<uint32_t>(blocks.size());
+    const uint32_t nb = static_cast<uint32_t>(blocks.size());
+    if (nb == 0 || nb > nb_max) return hipErrorInvalidValue;
+    const uint32_t nbp = (nb + 15) / 16 * 16;
+    if ((e = hipMemcpy(blocks_d.p, blocks.data(), nb * sizeof(PassBlock), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(first_d.p, first.data(), first.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(H, 0, static_cast<size_t>(kDigitBins) * nbp * 4)) != hipSuccess)
+      return e;
+    PassArgs a{};
+    a.planes = planes;
+    a.pos_in = pass == 1 ? nullptr : pos_buf[pass - 2];
+    a.r16_in = static_cast<const uint16_t*>(r16.p);
+    a.r8_in = static_cast<const uint8_t*>(r8.p);
+    a.pos_out = pos_buf[pass - 1];
+    a.r16_out = static_cast<uint16_t*>(r16.p);
+    a.r8_out = static_cast<uint8_t*>(r8.p);
+    a.blocks = static_cast<const PassBlock*>(blocks_d.p);
+    a.first = static_cast<const uint32_t*>(first_d.p);
+    a.seg_start = seg_d;
+    a.S = H;
+    a.nbp = nbp;
+    auto hist = pass == 1 ? pass_hist_kernel<1> : pass == 2 ? pass_hist_kernel<2> : pass_hist_kernel<3>;
+    auto place = pass == 1 ? pass_place_kernel<1> : pass == 2 ? pass_place_kernel<2> : pass_place_kernel<3>;
+    hipLaunchKernelGGL(hist, dim3(nb), dim3(kBuildThreads), 0, 0, a, H);
+    if ((e = exclusive_scan(H, static_cast<uint64_t>(kDigitBins) * nbp, tot)) != hipSuccess) return e;
+    hipLaunchKernelGGL(place, dim3(nb), dim3(kBuildThreads), 0, 0, a);
+    // starts of the next pass's segments (after pass 3: the bucket offsets)
+    uint32_t* next = pass == 3 ? offsets : pass == 1 ? static_cast<uint32_t*>(starts_a.p)
+                                                      : static_cast<uint32_t*>(starts_b.p);
+    hipLaunchKernelGGL(seg_starts_kernel, dim3(n_seg), dim3(kBuildThreads), 0, 0, a, n, n_seg, next);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    n_seg *= kDigitBins;
+    if (pass < 3) {
+      st.resize(n_seg + 1);
+      if ((e = hipMemcpy(st.data(), next, st.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+      seg_d = next;
+    }
   }
-  first[kDigitBins] = static_cast<uint32_t>(blocks.size());
-  const uint32_t nb2 = static_cast<uint32_t>(blocks.size());
-  if (nb2 == 0 || nb2 > 2ull * nb1 + kDigitBins) return hipErrorInvalidValue;
-  e = hipMalloc(&blocks_d.p, blocks.size() * sizeof(Pass2Block));
-  if (e == hipSuccess) e = hipMalloc(&first_d.p, first.size() * 4);
-  if (e == hipSuccess) e = hipMalloc(&h2.p, (static_cast<size_t>(kDigitBins) * nb2 + 1) * 4);
-  if (e == hipSuccess)
-    e = hipMemcpy(blocks_d.p, blocks.data(), blocks.size() * sizeof(Pass2Block), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(first_d.p, first.data(), first.size() * 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) return e;
-  auto* B2 = static_cast<const Pass2Block*>(blocks_d.p);
-  auto* F2 = static_cast<const uint32_t*>(first_d.p);
-  auto* H2 = static_cast<uint32_t*>(h2.p);
-  // pass 2: by the low 12 bits within each high digit
-  hipLaunchKernelGGL(lo_hist_kernel, dim3(nb2), dim3(kBuildThreads), 0, 0, L1, B2, nb2, H2);
-  if ((e = exclusive_scan(H2, static_cast<uint64_t>(kDigitBins) * nb2, tot)) != hipSuccess) return e;
-  hipLaunchKernelGGL(key_count_kernel, dim3(kBuckets / 256), dim3(256), 0, 0, H2, F2, nb2, offsets);
-  if ((e = exclusive_scan(offsets, kBuckets, tot)) != hipSuccess) return e;  // offsets[kBuckets] = n
-  hipLaunchKernelGGL(lo_place_kernel, dim3(nb2), dim3(kBuildThreads), 0, 0, P1, L1, B2, F2, nb2,
-                     H2, offsets, pos_out);
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 hipError_t build_index(const uint2* planes, uint64_t n, uint32_t* offsets, uint32_t* pos_out) {
-  DevBuf h1, totals, pos1, lo1, starts, blocks_d, first_d, h2;
-  const hipError_t e = build_passes(planes, n, offsets, pos_out, h1, totals, pos1, lo1, starts,
-                                    blocks_d, first_d, h2);
+  DevBuf h, totals, pos_tmp, r16, r8, blocks_d, first_d, starts_a, starts_b;
+  const hipError_t e = build_passes(planes, n, offsets, pos_out, h, totals, pos_tmp, r16, r8,
+                                    blocks_d, first_d, starts_a, starts_b);
   // no kernel may still use the temporaries when they are freed
   const hipError_t s = hipDeviceSynchronize();
   return e != hipSuccess ? e : s;
