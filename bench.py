@@ -76,13 +76,14 @@ def mfma_ops(families) -> int:
 # step loop, each scaled by the same counter's reading for a copy kernel of
 # known size.  Keyed by (workload, engine, max_mm): (bytes, the profile that
 # holds the counter runs and their commands).  None when not profiled.
-TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_666_711_636,
+TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_602_305_472,
                                         "profiles/batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md")}
 
 # Busy fractions of the units of the dominant kernel from rocprofv3 --pmc
-# (SQ_ACTIVE_INST_VALU, SQ_INST_CYCLES_VMEM..., SQ_VALU_MFMA_BUSY_CYCLES) on
-# the same workload, same profile as the traffic.
-UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.706, "mfma": 0.484}}
+# (VALU: 4 cycles per SQ_INSTS_VALU instruction; MFMA: SQ_VALU_MFMA_BUSY_CYCLES;
+# both over GRBM_GUI_ACTIVE SIMD-cycles) on the same workload, same profile as
+# the traffic.
+UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.696, "mfma": 0.502}}
 
 
 def site_keys(guide, strand, pos, n_mm) -> np.ndarray:
