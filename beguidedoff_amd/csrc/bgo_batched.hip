@@ -628,17 +628,23 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
       const v16f k3 = v16f{} + kField3Start;
       for (; ti + 2 < n_tiles; ti += 3) {
         uint32_t any = 0;
+        // two window tiles per step, their chains interleaved, so the matrix
+        // core has independent products back to back
+        static_assert(kAT % 2 == 0, "window tiles in pairs");
 #pragma unroll
-        for (int at = 0; at < static_cast<int>(kAT); ++at) {
+        for (int at = 0; at < static_cast<int>(kAT); at += 2) {
           v16f c = fp4_acc<0>(A[at], b0, k3);
+          v16f d = fp4_acc<0>(A[at + 1], b0, k3);
           c = fp4_acc<1>(A[at], b1, c);
-          if (at == static_cast<int>(kAT) - 1) b0 = btile(ti + 3);
+          d = fp4_acc<1>(A[at + 1], b1, d);
+          if (at + 2 == static_cast<int>(kAT)) b0 = btile(ti + 3);
           c = fp4_acc<2>(A[at], b2, c);
-          if (at == static_cast<int>(kAT) - 1) {
+          d = fp4_acc<2>(A[at + 1], b2, d);
+          if (at + 2 == static_cast<int>(kAT)) {
             b1 = btile(ti + 4);
             b2 = btile(ti + 5);
           }
-          any |= or16(c);
+          any |= or16(c) | or16(d);
           asm volatile("" : "+v"(any));
         }
         if (ballot((any & kField3Mask) != 0)) {
