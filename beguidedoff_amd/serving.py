@@ -17,14 +17,17 @@ dispatch of finished segments overlaps device work.
 Segment count: more segments shorten the wait to join but add a launch's
 start-up and tail to every request's pass (about 0.015 ms to start a
 persistent grid, plus the last work items).  On a 3.09 Gbp genome with
-1-500-guide requests at 2 and 32 req/s, 8 segments gave mean latencies of
-6.2-6.6 ms and p99 of 17.7-19.9 ms, against 6.5-7.2 and 19.1-21.4 ms for 4
-and 6.8-7.6 and 19.3-24.4 ms for 2.
+1-500-guide requests, same arrivals: p50 / p99 at 2 and 32 req/s were
+4.05 / 11.48 and 4.90 / 17.77 ms with 4 segments, 4.27 / 11.09 and
+4.79 / 16.53 ms with 8, 5.65 / 11.62 and 6.49 / 14.70 ms with 16.
 
 Large requests (`solo_guides`, off by default): requests of at least that
 many guides get launches of their own, taken in turn with the shared launch
 of the smaller requests, so a small request's launches never carry a large
-request's guides.  Each launch group keeps its own segment cursor.
+request's guides.  Each launch group keeps its own segment cursor.  With
+300 on the same 3.09 Gbp workload p99 did not fall (10.53 ms either way at
+2 req/s) and rose at 32 req/s (12.34 -> 15.92 ms): overlapping large
+requests in separate launches pay the per-window work twice.
 
 Admission: each request reserves an estimated number of hit records (from
 its PAMs and mismatch budgets); requests are admitted in arrival order while
