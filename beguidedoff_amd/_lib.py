@@ -84,6 +84,7 @@ _SIGNATURES = {
     "bgo_plan_graph_launch": (C.c_int, [_P, C.c_int32]),
     "bgo_plan_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "bgo_plan_family_stats": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P, _P]),
+    "bgo_plan_seed_stats": (C.c_int, [_P, C.c_int32, _P]),
     "bgo_scan_batched": (C.c_int, [_P, _P, C.c_int32, _P, C.POINTER(Tuning), _P, C.c_int64,
                                    C.POINTER(C.c_int64)]),
     "bgo_plan_assemble": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, C.c_int32, _P, _P, _P,

@@ -252,6 +252,13 @@ class Plan:
                                                      _lib.ptr(w), _lib.ptr(g)))
         return [(int(g[i]), int(w[i])) for i in range(nf.value)]
 
+    def seed_stats(self) -> List[int]:
+        """Seed engine: guide-table entries verified per family in the last launch."""
+        nf = len(self.family_stats())
+        v = np.zeros(max(nf, 1), dtype=np.int64)
+        _lib.check(_lib.load().bgo_plan_seed_stats(self.handle, nf, _lib.ptr(v)))
+        return [int(x) for x in v[:nf]]
+
     def close(self) -> None:
         if self._h is not None:
             _lib.load().bgo_plan_destroy(self._h)

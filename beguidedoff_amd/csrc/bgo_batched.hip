@@ -63,6 +63,8 @@ struct Fam {
   uint32_t kmax;            // largest budget among the family's guides
   uint32_t g_live;          // end of the family's real (unpadded) guides
   uint32_t direct;          // 1: windows compared in place, without the buffer
+  uint32_t seed_start;      // seed engine: word offset of the key starts in the block's table
+  uint32_t seed_ent;        // seed engine: word offset of the entries in the block's table
   uint32_t pad;
 };
 
@@ -75,7 +77,8 @@ struct Args {
   uint32_t valid_begin, valid_end;
   uint32_t chunk, chunks_per_slot, n_items, n_fam;
   uint32_t n_slots, interleave;  // interleave: consecutive items alternate strands
-  uint32_t btab_bytes, pad_;     // size of the MFMA guide operand table
+  uint32_t btab_bytes;           // size of the MFMA guide operand table (seed engine: of the group's table)
+  uint32_t fam_total;            // families of the plan: per-family verify counts follow the window counts
   unsigned long long hit_cap;
 };
 
@@ -93,8 +96,10 @@ struct Tab {
   bgo_hit* __restrict__ hits;
   unsigned long long* __restrict__ hit_count;
   const int4* __restrict__ btab;         // MFMA engines: guide operand tiles
-  bgo_hit* rec_stage;                    // MFMA engines: the wave's staged records (LDS), else null
+  bgo_hit* rec_stage;                    // MFMA and seed engines: the wave's staged records (LDS), else null
   uint32_t* rec_cnt;                     // records staged (LDS)
+  const uint32_t* stab;                  // seed engine: the block's seed table (LDS)
+  const uint32_t* __restrict__ sgix;     // seed engine: guide of each table word
 };
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
@@ -114,7 +119,8 @@ __host__ __device__ constexpr uint32_t ring_size(int engine) { return ring_flush
 __host__ __device__ constexpr size_t wave_lds_bytes(int nf, int engine) {
   return static_cast<size_t>(nf) * ring_size(engine) * 8 + kMaxFam * 8 +
          static_cast<size_t>(nf) * 64 * 4 + kMaxFam * 4 + 64 * 8 +
-         (engine >= 2 ? kRecStage * sizeof(bgo_hit) + 16 : 0);
+         (engine >= 2 ? kRecStage * sizeof(bgo_hit) + 16 : 0) +
+         (engine == 4 ? static_cast<size_t>(nf) * 64 * 4 : 0);
 }
 
 __device__ inline uint32_t slot_pos(uint2 e) { return (e.x >> 20) | ((e.y >> 20) << 12); }
@@ -705,6 +711,96 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
   }
 }
 
+// Seed engine (ENGINE 4): a window within 4 mismatches of a guide matches it
+// exactly in at least one of the five 4-base blocks of the 20 compared bases
+// (pigeonhole), so each window looks up, per block, the family's guides with
+// the same 4 bases there and verifies only those.  Per family and block j the
+// block's seed table (shared by the waves of a block, in LDS) holds 257 key
+// starts and then, sorted by key (the 8 plane bits of the block's 4 bases),
+// one word per guide: its other 16 bases, b0 plane bits low and b1 plane bits
+// high.  A table word's guide is in sgix at the same word offset.  Only
+// families whose budgets are all <= 4 use it.
+constexpr uint32_t kSeedBlocks = 5;
+constexpr uint32_t kSeedKeys = 256;
+constexpr uint32_t kSeedPad = 4;  // spare words after a family's entries
+
+// The 16 bits of a 20-bit plane word outside bases 4j..4j+3, packed in order.
+__host__ __device__ inline uint32_t seed_other(uint32_t w, uint32_t j) {
+  return (w & ((1u << (4 * j)) - 1u)) | ((w >> (4 * j + 4)) << (4 * j));
+}
+
+// Exact path of a seed candidate: the guide behind table word widx, its full
+// 2-bit count, and the pair only from the first block in which it matches
+// exactly (so a pair matching in several blocks is recorded once).
+__device__ inline void seed_exact(const Args& a, const Tab& t, uint32_t s, uint32_t lane,
+                                  uint32_t W0, uint32_t W1, uint32_t pos, uint32_t j,
+                                  uint32_t widx, bool cand) {
+  uint32_t gi = 0, c = 0;
+  bool rec = false;
+  if (cand) {
+    gi = t.sgix[widx];
+    const uint32_t m = ((W0 ^ t.g01[2 * gi]) | (W1 ^ t.g01[2 * gi + 1])) & kWin;
+    uint32_t nz = m | (m >> 1);
+    nz |= nz >> 2;  // bit 4b: block b has a mismatch
+    const uint32_t below = 0x11111u & ((1u << (4 * j)) - 1u);
+    c = static_cast<uint32_t>(__popc(m));
+    rec = (nz & below) == below && static_cast<int>(c) <= t.gk[gi];
+  }
+  if (ballot(rec)) record_candidates(a, t, s, gi, pos, W0, W1, c, rec, lane);
+}
+
+// Compare the first n (<= 64) windows of a family ring with the family's
+// guides through the seed table: one window per lane; per block, each lane
+// walks its own key's guides, the wave as long as its longest walk.
+__device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, uint32_t s,
+                                    const uint2* L, uint32_t n, uint32_t cs, uint32_t lane,
+                                    uint32_t* vcnt) {
+  uint32_t verified = 0;
+  const bool has = lane < n;
+  const uint2 e = has ? L[lane] : make_uint2(0u, 0u);
+  const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
+  const uint32_t pos = cs + slot_pos(e);
+  const uint32_t kmax = F.kmax;
+  const uint32_t n_g = F.g_live - F.g_begin;
+  const uint32_t* st = t.stab + F.seed_start;
+  const uint32_t* en = t.stab + F.seed_ent;
+#pragma unroll
+  for (uint32_t j = 0; j < kSeedBlocks; ++j) {
+    const uint32_t key = ((W0 >> (4 * j)) & 0xFu) | (((W1 >> (4 * j)) & 0xFu) << 4);
+    const uint32_t wo = seed_other(W0, j) | (seed_other(W1, j) << 16);
+    const uint32_t* sj = st + j * (kSeedKeys + 1);
+    const uint32_t beg = sj[key];
+    const uint32_t end = has ? sj[key + 1] : beg;
+    const uint32_t* ej = en + j * n_g;
+    verified += end - beg;
+    // Four entries per step, read together (each family's entries end with
+    // kSeedPad spare words, so the reads past a lane's list stay in the
+    // table; their results are masked).
+    for (uint32_t i = beg; ballot(i < end); i += 4) {
+      uint32_t x[4];
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) x[q] = ej[i + q];
+      uint32_t hit = 0;  // bit q: entry i + q is a candidate
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t d = x[q] ^ wo;
+        const uint32_t m = (d | (d >> 16)) & 0xFFFFu;
+        const uint32_t ok = static_cast<uint32_t>(i + q < end) &
+                            static_cast<uint32_t>(static_cast<uint32_t>(__popc(m)) <= kmax);
+        hit |= ok << q;
+      }
+      if (ballot(hit != 0)) {
+#pragma nounroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          const bool cand = (hit >> q) & 1u;
+          if (ballot(cand)) seed_exact(a, t, s, lane, W0, W1, pos, j, F.seed_ent + j * n_g + i + q, cand);
+        }
+      }
+    }
+  }
+  vcnt[lane] += verified;
+}
+
 // Per-wave LDS state of the scan loop.
 struct WaveState {
   uint2* lists;                  // [NF][ring_size] window buffers
@@ -712,6 +808,7 @@ struct WaveState {
   uint32_t* counts;              // [kMaxFam] windows in each buffer
   unsigned long long* wstats;    // [kMaxFam] windows compared
   uint2* stage;                  // [64] plane words of the next step's blocks
+  uint32_t* vcnt;                // seed engine: [NF][64] table entries each lane verified
   template <int ENGINE>
   __device__ uint2* ring(uint32_t f) const {
     return lists + f * ring_size(ENGINE);
@@ -725,8 +822,14 @@ __device__ inline void flush_front(const Args& a, const Tab& t, const WaveState&
                                    uint32_t s, uint2* L, uint32_t cb, uint32_t lane) {
   constexpr uint32_t kFlush = ring_flush(ENGINE);
   wave_sync();
-  if constexpr (ENGINE >= 2)
+  if constexpr (ENGINE == 2 || ENGINE == 3)
     compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, kFlush, cb, lane);
+  else if constexpr (ENGINE == 4)
+  {
+    __builtin_amdgcn_s_setprio(0);
+    seed_compare(a, t, t.fam[f], s, L, kFlush, cb, lane, w.vcnt + f * 64);
+    __builtin_amdgcn_s_setprio(1);
+  }
   else
   {
     // VALU batch compare at wave priority 0, like the fp4 full-batch compare
@@ -880,8 +983,14 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
       c -= ring_flush(ENGINE);
     }
     if (c) {
-      if constexpr (ENGINE >= 2)
+      if constexpr (ENGINE == 2 || ENGINE == 3)
         compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane);
+      else if constexpr (ENGINE == 4)
+      {
+        __builtin_amdgcn_s_setprio(0);
+        seed_compare(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane, w.vcnt + f * 64);
+        __builtin_amdgcn_s_setprio(1);
+      }
       else
       {
         __builtin_amdgcn_s_setprio(0);
@@ -895,22 +1004,37 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
 }
 
 template <int NF, int ENGINE>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void batched_scan_kernel(
-    Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
+__device__ __forceinline__ void scan_kernel_body(
+    uint2* lds, Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
     const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
     const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gnm,
     const double* __restrict__ score_tab, bgo_hit* __restrict__ hits,
     unsigned long long* __restrict__ hit_count, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ fam_windows, const int4* __restrict__ btab) {
-  extern __shared__ uint2 lds[];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t n_waves = blockDim.x >> 6;
+  // Seed engine: the block's seed table (a.btab_bytes, a multiple of 16, at
+  // btab, followed by its guide words) is copied to the front of the LDS by
+  // the whole block; the waves' own regions follow it.
+  size_t lds_front = 0;
+  const uint32_t* stab = nullptr;
+  const uint32_t* sgix = nullptr;
+  if constexpr (ENGINE == 4) {
+    const uint32_t n16 = a.btab_bytes / 16;
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    const uint4* src = reinterpret_cast<const uint4*>(btab);
+    for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+    stab = reinterpret_cast<const uint32_t*>(lds);
+    sgix = reinterpret_cast<const uint32_t*>(btab) + a.btab_bytes / 4;
+    lds_front = (static_cast<size_t>(a.btab_bytes) + 255) / 256 * 256;
+  }
   // LDS per wave: NF window buffers (ring_size x 8 B), kMaxFam stats (8 B),
   // NF*64 pass words, kMaxFam buffer counts (4 B each), 64 staged plane
   // words (8 B each).
   constexpr size_t kRings = NF * ring_size(ENGINE) * 8;
-  char* my = reinterpret_cast<char*>(lds) + wave * wave_lds_bytes(NF, ENGINE);
+  char* my = reinterpret_cast<char*>(lds) + lds_front + wave * wave_lds_bytes(NF, ENGINE);
   WaveState w;
   w.lists = reinterpret_cast<uint2*>(my);
   w.wstats = reinterpret_cast<unsigned long long*>(my + kRings);
@@ -923,12 +1047,21 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
       ENGINE >= 2 ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(w.stage + 64) +
                                                 kRecStage * sizeof(bgo_hit))
                   : nullptr;
-  const Tab t{fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count, btab, rec_stage, rec_cnt};
+  // seed engine: per-lane verify counters after the record stage
+  w.vcnt = ENGINE == 4 ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(w.stage + 64) +
+                                                     kRecStage * sizeof(bgo_hit) + 16)
+                       : nullptr;
+  if constexpr (ENGINE == 4) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) w.vcnt[f * 64 + lane] = 0;
+  }
+  const Tab t{fam,       g01,       gk,        gaux, gid,     gnm,  score_tab,
+              hits,      hit_count, btab,      rec_stage, rec_cnt, stab, sgix};
   // fp4 and VALU: priority 1 outside the batch compares, which run at 0 (the
   // scan, append and in-place passes are dependent chains; a compare has
   // many independent steps in flight).  VALU: 22 / 80 / 1024 mixed guides
   // 3.38 / 5.83 / 37.8 -> 3.27 / 5.43 / 35.8 ms.
-  if constexpr (ENGINE == 3 || ENGINE == 1) __builtin_amdgcn_s_setprio(1);
+  if constexpr (ENGINE == 3 || ENGINE == 1 || ENGINE == 4) __builtin_amdgcn_s_setprio(1);
   if (lane < kMaxFam) w.wstats[lane] = 0;
   if constexpr (ENGINE >= 2) {
     if (lane == 0) *rec_cnt = 0;
@@ -969,6 +1102,45 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
   if constexpr (ENGINE >= 2) flush_records(a, t, lane);
   wave_sync();
   if (lane < NF && w.wstats[lane]) atomicAdd(&fam_windows[lane], w.wstats[lane]);
+  if constexpr (ENGINE == 4) {
+    // one atomic per wave and family: the lanes' counts summed across the wave
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      unsigned long long v = w.vcnt[f * 64 + lane];
+#pragma unroll
+      for (int sh = 32; sh >= 1; sh >>= 1) v += __shfl_xor(v, sh, 64);
+      if (lane == 0 && v) atomicAdd(&fam_windows[a.fam_total + f], v);
+    }
+  }
+}
+
+template <int NF, int ENGINE>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) void batched_scan_kernel(
+    Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
+    const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
+    const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gnm,
+    const double* __restrict__ score_tab, bgo_hit* __restrict__ hits,
+    unsigned long long* __restrict__ hit_count, unsigned int* __restrict__ work_counter,
+    unsigned long long* __restrict__ fam_windows, const int4* __restrict__ btab) {
+  extern __shared__ uint2 lds[];
+  scan_kernel_body<NF, ENGINE>(lds, a, fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count,
+                               work_counter, fam_windows, btab);
+}
+
+// The seed engine's kernel: one block of kSeedBlock threads per CU shares the
+// seed table in LDS.
+constexpr int kSeedBlock = 1024;
+template <int NF>
+__global__ __launch_bounds__(kSeedBlock) void seed_scan_kernel(
+    Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
+    const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
+    const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gnm,
+    const double* __restrict__ score_tab, bgo_hit* __restrict__ hits,
+    unsigned long long* __restrict__ hit_count, unsigned int* __restrict__ work_counter,
+    unsigned long long* __restrict__ fam_windows, const int4* __restrict__ btab) {
+  extern __shared__ uint2 lds[];
+  scan_kernel_body<NF, 4>(lds, a, fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count,
+                          work_counter, fam_windows, btab);
 }
 
 using KernelFn = void (*)(Args, const Fam*, const uint32_t*, const int32_t*, const uint32_t*,
@@ -989,7 +1161,19 @@ inline KernelFn kernel_for_nf(int nf) {
   }
 }
 
+// Seed engine launches group at most kSeedMaxFam families (their rings and
+// the shared table share one block's LDS).
+constexpr int kSeedMaxFam = 4;
+
 inline KernelFn kernel_for(int nf, int engine) {
+  if (engine == 4) {
+    switch (nf) {
+      case 1: return seed_scan_kernel<1>;
+      case 2: return seed_scan_kernel<2>;
+      case 3: return seed_scan_kernel<3>;
+      default: return seed_scan_kernel<4>;
+    }
+  }
   return engine == 3 ? kernel_for_nf<3>(nf) : engine == 2 ? kernel_for_nf<2>(nf) : kernel_for_nf<1>(nf);
 }
 
@@ -1060,6 +1244,8 @@ struct bgo_plan {
   int resolved_blocks_per_cu = 0;
   std::vector<int> group_blocks;     // persistent blocks per launch group
   std::vector<size_t> group_lds;     // dynamic LDS bytes per launch group
+  std::vector<size_t> group_img_off;    // seed engine: byte offset of each group's table in d_btab
+  std::vector<size_t> group_img_bytes;  // seed engine: bytes of each group's table (LDS copy)
   int threads = 0;
   // Tables and hit buffer in memory the plan does not own (the no-alloc
   // entry point's scratch array): never grown, never freed.
@@ -1225,8 +1411,8 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   P->direct_off = r.reserved[1] < 0;
   P->engine_auto = r.engine <= 0;
   if (r.engine <= 0) r.engine = 1;
-  if (r.engine < 1 || r.engine > 3) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
-  r.guide_unroll = r.engine >= 2 ? 32 : bk::kUnroll;
+  if (r.engine < 1 || r.engine > 4) return fail(BGO_ERR_ARG, "engine %d not available", r.engine);
+  r.guide_unroll = r.engine == 4 ? 1 : r.engine >= 2 ? 32 : bk::kUnroll;
   P->tune = r;
   return BGO_OK;
 }
@@ -1234,14 +1420,22 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
 int plan_configure_launch(bgo_plan* P) {
   int n_cu = 0;
   BGO_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, P->device));
-  P->threads = P->tune.waves_per_block * 64;
+  const bool seed = P->tune.engine == 4;
+  P->threads = seed ? bk::kSeedBlock : P->tune.waves_per_block * 64;
   P->group_blocks.clear();
   P->group_lds.clear();
   int min_occ = 1 << 30;
   for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
-    const int nf = std::min<int>(bk::kMaxFam, static_cast<int>(P->fams.size()) - P->group_first[gi]);
+    const int f_end = gi + 1 < P->group_first.size() ? P->group_first[gi + 1]
+                                                     : static_cast<int>(P->fams.size());
+    const int nf = f_end - P->group_first[gi];
     const size_t per_wave = bk::wave_lds_bytes(nf, P->tune.engine);
-    const size_t lds = per_wave * P->tune.waves_per_block;
+    const size_t lds = seed ? align256(P->group_img_bytes[gi]) + per_wave * (bk::kSeedBlock / 64)
+                            : per_wave * P->tune.waves_per_block;
+    if (seed)
+      BGO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(bk::kernel_for(nf, 4)),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds)));
     int occ = 0;
     BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bk::kernel_for(nf, P->tune.engine), P->threads,
                                                          lds));
@@ -1252,8 +1446,8 @@ int plan_configure_launch(bgo_plan* P) {
     // Wave w takes items from work counter w % kWorkCtrs only, so the grid
     // needs a wave for every counter (blocks past occupancy start later and
     // still drain their counter).
-    const int min_blocks = (static_cast<int>(bk::kWorkCtrs) + P->tune.waves_per_block - 1) /
-                           P->tune.waves_per_block;
+    const int wpb = P->threads / 64;
+    const int min_blocks = (static_cast<int>(bk::kWorkCtrs) + wpb - 1) / wpb;
     P->group_blocks.push_back(std::max(occ * n_cu, min_blocks));
     P->group_lds.push_back(lds);
   }
@@ -1293,6 +1487,77 @@ int validate_guides(const bgo_guide_desc* guides, int32_t n_guides) {
 // Group guides by PAM family (order of first appearance), build the padded
 // tables, grow the device allocation if needed and upload on the plan stream.
 // The previous launch must be complete (callers synchronise first).
+// Seed engine tables.  A family part holds at most kSeedMaxGuides guides,
+// so that its table (5 x 257 key starts, 5 words per guide and 4 spare
+// words, 87,076 bytes at 4,096 guides) and one family's wave buffers for a 1,024-thread block fit
+// the 160 KiB of LDS of a CU.
+constexpr size_t kSeedMaxGuides = 4096;
+constexpr size_t kSeedLdsBytes = 160 * 1024;
+
+size_t seed_image_bytes(uint32_t n_guides) {
+  return (bk::kSeedBlocks * (bk::kSeedKeys + 1) + bk::kSeedBlocks * size_t{n_guides} + bk::kSeedPad) * 4;
+}
+
+// LDS of a seed-engine block: the table (its start rounded to 256 bytes for
+// the wave buffers after it) and the buffers of its 16 waves.
+size_t seed_block_lds(size_t image_bytes, int nf) {
+  return (image_bytes + 255) / 256 * 256 + bk::wave_lds_bytes(nf, 4) * (bk::kSeedBlock / 64);
+}
+
+// Per launch group: the image copied to LDS (per family, then per block j of
+// the 20 compared bases: 257 key starts, then one word per guide sorted by
+// the 8-bit key of its bases 4j..4j+3: the other 16 bases, b0 plane bits low
+// and b1 high), padded to 16 bytes, followed by as many words naming the
+// guide behind each entry word.  Fam::seed_start / seed_ent are word offsets
+// into the group's image.
+int build_seed_tables(bgo_plan* P) {
+  P->h_btab.clear();
+  for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
+    const int f0 = P->group_first[gi];
+    const int f_end = gi + 1 < P->group_first.size() ? P->group_first[gi + 1]
+                                                     : static_cast<int>(P->fams.size());
+    std::vector<uint32_t> img, gix;
+    for (int f = f0; f < f_end; ++f) {
+      Fam& F = P->fams[f];
+      const uint32_t n = F.g_live - F.g_begin;
+      F.seed_start = static_cast<uint32_t>(img.size());
+      F.seed_ent = F.seed_start + bk::kSeedBlocks * (bk::kSeedKeys + 1);
+      img.resize(F.seed_ent + bk::kSeedBlocks * n + bk::kSeedPad, 0u);
+      gix.resize(img.size(), 0u);
+      for (uint32_t j = 0; j < bk::kSeedBlocks; ++j) {
+        uint32_t* st = img.data() + F.seed_start + j * (bk::kSeedKeys + 1);
+        uint32_t cnt[bk::kSeedKeys] = {};
+        auto key_of = [&](uint32_t g) {
+          const uint32_t p0 = P->h_g01[2 * g], p1 = P->h_g01[2 * g + 1];
+          return ((p0 >> (4 * j)) & 0xFu) | (((p1 >> (4 * j)) & 0xFu) << 4);
+        };
+        for (uint32_t g = F.g_begin; g < F.g_live; ++g) ++cnt[key_of(g)];
+        st[0] = 0;
+        for (uint32_t k = 0; k < bk::kSeedKeys; ++k) st[k + 1] = st[k] + cnt[k];
+        uint32_t fill[bk::kSeedKeys];
+        for (uint32_t k = 0; k < bk::kSeedKeys; ++k) fill[k] = st[k];
+        for (uint32_t g = F.g_begin; g < F.g_live; ++g) {  // stable: guide order within a key
+          const uint32_t w = F.seed_ent + j * n + fill[key_of(g)]++;
+          img[w] = bk::seed_other(P->h_g01[2 * g], j) | (bk::seed_other(P->h_g01[2 * g + 1], j) << 16);
+          gix[w] = g;
+        }
+      }
+    }
+    img.resize((img.size() + 3) / 4 * 4, 0u);
+    gix.resize(img.size(), 0u);
+    const size_t off = (P->h_btab.size() + 255) / 256 * 256;
+    const size_t bytes = img.size() * 4;
+    if (seed_block_lds(bytes, f_end - f0) > kSeedLdsBytes)
+      return fail(BGO_ERR_ARG, "seed table of %zu bytes does not fit a block's LDS", bytes);
+    P->h_btab.resize(off + 2 * bytes, 0);
+    std::memcpy(P->h_btab.data() + off, img.data(), bytes);
+    std::memcpy(P->h_btab.data() + off + bytes, gix.data(), bytes);
+    P->group_img_off.push_back(off);
+    P->group_img_bytes.push_back(bytes);
+  }
+  return BGO_OK;
+}
+
 int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides) {
   BGO_TRY(validate_guides(guides, n_guides));
   std::map<uint64_t, int> fam_index;
@@ -1308,7 +1573,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     }
     members[it->second].push_back(i);
   }
-  const int n_fam = static_cast<int>(members.size());
+  int n_fam = static_cast<int>(members.size());
   if (P->engine_auto) {
     // The engine with the lower modelled cost.  Per family, in ms per unit of
     // PAM pass rate on a 3.09 Gbp genome (fitted to single-family and mixed
@@ -1325,6 +1590,26 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     P->tune.engine = fp4 < valu ? 3 : 1;
     P->tune.guide_unroll = P->tune.engine >= 2 ? 32 : bk::kUnroll;
   }
+  if (P->tune.engine == 4) {
+    // The seed engine's pigeonhole (five exact 4-base blocks) covers budgets
+    // up to 4; a family's table holds at most kSeedMaxGuides guides, so
+    // larger families are split (the parts scan in launches of their own).
+    for (int32_t i = 0; i < n_guides; ++i)
+      if (guides[i].max_mm > 4)
+        return fail(BGO_ERR_ARG, "engine 4 (seed) needs budgets <= 4; guide %d has %d", i,
+                    guides[i].max_mm);
+    std::vector<std::vector<int32_t>> parts;
+    std::vector<Fam> part_proto;
+    for (int f = 0; f < n_fam; ++f)
+      for (size_t b = 0; b < members[f].size(); b += kSeedMaxGuides) {
+        const size_t e = std::min(members[f].size(), b + kSeedMaxGuides);
+        parts.emplace_back(members[f].begin() + b, members[f].begin() + e);
+        part_proto.push_back(proto[f]);
+      }
+    members.swap(parts);
+    proto.swap(part_proto);
+  }
+  const int n_fam_parts = static_cast<int>(members.size());
   P->fams.clear();
   P->fam_guides.clear();
   P->group_first.clear();
@@ -1338,7 +1623,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   auto& gid = P->h_gid;
   auto& gnm = P->h_gnm;
   auto& gk = P->h_gk;
-  for (int f = 0; f < n_fam; ++f) {
+  for (int f = 0; f < n_fam_parts; ++f) {
     Fam F = proto[f];
     F.g_begin = static_cast<uint32_t>(gk.size());
     uint32_t kmax = 0;
@@ -1361,7 +1646,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       gnm.push_back(eqT);
       kmax = std::max<uint32_t>(kmax, d.max_mm);
     }
-    const size_t pad_to = P->tune.engine >= 2 ? 32 : bk::kUnroll;
+    const size_t pad_to = P->tune.engine == 4 ? 1 : P->tune.engine >= 2 ? 32 : bk::kUnroll;
     while ((gk.size() - F.g_begin) % pad_to) {  // padding guides never match
       g01.push_back(0xFFF00000u);
       g01.push_back(0xFFF00000u);
@@ -1379,7 +1664,26 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     P->fams.push_back(F);
     P->fam_guides.push_back(static_cast<int32_t>(members[f].size()));
   }
-  for (int f = 0; f < n_fam; f += bk::kMaxFam) P->group_first.push_back(f);
+  if (P->tune.engine == 4) {
+    // Launch groups of families whose seed tables and wave buffers fit one
+    // block's LDS together.
+    int f = 0;
+    while (f < n_fam_parts) {
+      P->group_first.push_back(f);
+      size_t img = seed_image_bytes(P->fams[f].g_live - P->fams[f].g_begin);
+      int nf = 1;
+      while (f + nf < n_fam_parts && nf < bk::kSeedMaxFam) {
+        const size_t more = img + seed_image_bytes(P->fams[f + nf].g_live - P->fams[f + nf].g_begin);
+        if (seed_block_lds(more, nf + 1) > kSeedLdsBytes) break;
+        img = more;
+        ++nf;
+      }
+      f += nf;
+    }
+  } else {
+    for (int f = 0; f < n_fam_parts; f += bk::kMaxFam) P->group_first.push_back(f);
+  }
+  n_fam = n_fam_parts;
   const size_t n_pad = gk.size();
   P->n_guides = n_guides;
 
@@ -1447,9 +1751,13 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       }
     }
   }
+  P->group_img_off.clear();
+  P->group_img_bytes.clear();
+  if (P->tune.engine == 4) BGO_TRY(build_seed_tables(P));
   const size_t btab_bytes = P->h_btab.size();
 
-  P->work_word = (1 + n_fam + 31) / 32 * 32;
+  // [hit count][windows per family][seed verifies per family][pad to 256 B][work counters]
+  P->work_word = (1 + 2 * n_fam + 31) / 32 * 32;
   P->n_counter_words = P->work_word + static_cast<uint32_t>(P->group_first.size()) *
                                          bk::kWorkCtrs * (bk::kCtrStride / 2);
   size_t off = 0;
@@ -1605,9 +1913,12 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
   const uint32_t chunk = static_cast<uint32_t>(P->tune.chunk_positions);
   const uint64_t cps = (static_cast<uint64_t>(ve - vb) + chunk - 1) / chunk;
   if (cps * n_slots >= 0xFFFFFFF0ull) return fail(BGO_ERR_ARG, "too many work items");
+  const bool seed = P->tune.engine == 4;
   for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
     const int f0 = P->group_first[gi];
-    const int nf = std::min<int>(bk::kMaxFam, static_cast<int>(P->fams.size()) - f0);
+    const int f_end = gi + 1 < P->group_first.size() ? P->group_first[gi + 1]
+                                                     : static_cast<int>(P->fams.size());
+    const int nf = f_end - f0;
     bk::Args a{};
     a.planes0 = G->d_planes[0];
     a.planes1 = G->d_planes[1];
@@ -1623,15 +1934,17 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
     a.n_slots = n_slots;
     a.interleave = P->tune.reserved[0] ? 1u : 0u;
     a.n_fam = static_cast<uint32_t>(nf);
-    a.btab_bytes = static_cast<uint32_t>(P->h_btab.size());
+    a.fam_total = static_cast<uint32_t>(P->fams.size());
+    a.btab_bytes = static_cast<uint32_t>(seed ? P->group_img_bytes[gi] : P->h_btab.size());
     a.hit_cap = static_cast<unsigned long long>(P->hit_cap);
+    const int8_t* btab = seed ? P->d_btab + P->group_img_off[gi] : P->d_btab;
     hipLaunchKernelGGL(bk::kernel_for(nf, P->tune.engine), dim3(P->group_blocks[gi]), dim3(P->threads),
                        P->group_lds[gi], P->stream, a, P->d_fam + f0, P->d_g01, P->d_gk, P->d_gaux, P->d_gid,
                        P->d_gnm, P->d_score, P->d_hits, P->d_counters,
                        reinterpret_cast<unsigned int*>(P->d_counters + P->work_word) +
                            gi * bk::kWorkCtrs * bk::kCtrStride,
                        P->d_counters + 1 + f0,
-                       reinterpret_cast<const int4*>(P->d_btab));
+                       reinterpret_cast<const int4*>(btab));
     BGO_HIP(hipGetLastError());
   }
   return BGO_OK;
@@ -1778,6 +2091,21 @@ extern "C" int bgo_plan_family_stats(const bgo_plan* P, int32_t cap, int32_t* n_
     if (windows_out) windows_out[f] = static_cast<int64_t>(w[f]);
     if (guides_out) guides_out[f] = P->fam_guides[f];
   }
+  return BGO_OK;
+}
+
+extern "C" int bgo_plan_seed_stats(const bgo_plan* P, int32_t cap, int64_t* verified_out) {
+  if (!P || (cap > 0 && !verified_out)) return fail(BGO_ERR_ARG, "null pointer");
+  const int n_fam = static_cast<int>(P->fams.size());
+  if (cap <= 0 || n_fam == 0) return BGO_OK;
+  BGO_TRY(injected());
+  std::vector<unsigned long long> v(n_fam);
+  BGO_HIP(hipSetDevice(P->device));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  BGO_HIP(hipMemcpyAsync(v.data(), P->d_counters + 1 + n_fam, n_fam * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, P->stream));
+  BGO_HIP(hipStreamSynchronize(P->stream));
+  for (int f = 0; f < std::min(cap, n_fam); ++f) verified_out[f] = static_cast<int64_t>(v[f]);
   return BGO_OK;
 }
 

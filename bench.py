@@ -60,7 +60,20 @@ def algorithmic_ops(L_pos: int, n_strands: int, families) -> tuple:
     return ops, survey
 
 
-ENGINE_NAMES = {1: "batched VALU", 2: "batched int8 MFMA", 3: "batched fp4 MFMA"}
+ENGINE_NAMES = {1: "batched VALU", 2: "batched int8 MFMA", 3: "batched fp4 MFMA",
+                4: "batched seed-and-verify (VALU)"}
+
+
+def seed_ops(L_pos: int, n_strands: int, families, verified) -> int:
+    """32-bit lane-ops of the seed-and-verify formulation for one launch: the
+    window and PAM extraction and PAM tests of the bit-plane scan, per
+    (window, 4-base block) the block's key and the window's other 16 bases
+    (10 ops), and per verified guide-table entry xor, fold, mask, popcount and
+    compare (5 ops); `verified` counts the entries on the device."""
+    ops = n_strands * L_pos * 2 + sum(n_strands * L_pos * 4 for _ in families)
+    ops += sum(5 * 10 * w for _, w in families)
+    ops += 5 * int(sum(verified))
+    return ops
 
 
 def mfma_ops(families) -> int:
@@ -647,6 +660,10 @@ def main() -> int:
         bound, unit, dtype = "mfma", "TFLOP/s", "int (exact); fp4 e2m1 MFMA operands"
         alg = mfma_ops(families)
         peak = 10.1e3
+    elif engine == 4:  # seed-and-verify: its own lane-ops against the VALU issue peak
+        bound, unit, dtype = "valu", "Tops/s", "u32 (exact)"
+        alg = seed_ops(L_pos, 2, families, plan.seed_stats())
+        peak = peaks["valu_lane_ops"] / 1e12
     else:            # VALU compare: 32-bit lane-ops against the VALU issue peak
         bound, unit, dtype = "valu", "Tops/s", "u32 (exact)"
         alg = ops
@@ -657,7 +674,10 @@ def main() -> int:
 
     # both compare engines on the same workload (same tiling otherwise)
     engines = {}
-    for eng, name in () if args.no_engines else ((1, "valu"), (2, "mfma_i8"), (3, "mfma_fp4")):
+    legs = ((1, "valu"), (2, "mfma_i8"), (3, "mfma_fp4"))
+    if args.max_mm <= 4:
+        legs += ((4, "seed"),)
+    for eng, name in () if args.no_engines else legs:
         t_eng = dict(plan.tuning())
         t_eng["engine"] = eng
         p2 = Plan(dg, specs, tuning=t_eng, hit_capacity=default_capacity(len(specs)))
@@ -688,6 +708,9 @@ def main() -> int:
                                    alg, families, survey_ops, L, kms, peaks),
     }
 
+    # the batched site set, as a hash of its sorted (guide, strand, pos, n_mm) keys
+    bk_keys = np.sort(site_keys(hits["guide"], hits["strand"], hits["pos"], hits["n_mm"]))
+    out["sites_sha256"] = hashlib.sha256(bk_keys.tobytes()).hexdigest()[:16]
     if not args.no_plain:
         from beguidedoff_amd.cuda_search import last_kernel_ms, search_genome_cuda
         from beguidedoff_amd.encoding import encode_dna, encode_guide_64bit
@@ -726,9 +749,7 @@ def main() -> int:
                         "steps": args.plain_steps}
         out["speedup_vs_plain"] = round(pk / kms, 2)
         pk_keys = np.sort(np.concatenate(plain_keys)) if plain_keys else np.zeros(0, np.uint64)
-        bk_keys = np.sort(site_keys(hits["guide"], hits["strand"], hits["pos"], hits["n_mm"]))
         out["sites_equal_plain"] = bool(np.array_equal(pk_keys, bk_keys))
-        out["sites_sha256"] = hashlib.sha256(bk_keys.tobytes()).hexdigest()[:16]
 
     if args.rates:
         rates = [float(r) for r in args.rates.split(",") if r]

@@ -160,10 +160,12 @@ typedef struct bgo_tuning {
   int32_t chunk_positions;   /* positions per work item (multiple of 64) */
   int32_t waves_per_block;   /* 1, 2 or 4 */
   int32_t blocks_per_cu;     /* resident blocks requested per CU */
-  int32_t engine;            /* 0 auto (VALU or fp4 MFMA, whichever a per-family
-                                cost model of the guide counts and PAM pass
-                                rates favours), 1 VALU xor/popcount,
-                                2 int8 MFMA, 3 fp4 MFMA */
+  int32_t engine;            /* 0 auto (VALU, fp4 MFMA or seed, whichever a
+                                per-family cost model of the guide counts and
+                                PAM pass rates favours), 1 VALU xor/popcount,
+                                2 int8 MFMA, 3 fp4 MFMA, 4 seed-and-verify
+                                (VALU over LDS tables of the guides' 4-base
+                                blocks; every budget must be <= 4) */
   int32_t guide_unroll;      /* output: guides per compare step (8 VALU, 32 MFMA) */
   int32_t reserved[3];       /* [0]: 1 = work items alternate strands;
                                 [1]: VALU engine, PAM families of at most this
@@ -243,6 +245,14 @@ int bgo_plan_get_tuning(const bgo_plan* p, bgo_tuning* out);
 int bgo_plan_family_stats(const bgo_plan* p, int32_t cap, int32_t* n_fam_out,
                           int64_t* windows_out /*[cap]*/,
                           int32_t* guides_out /*[cap]*/);
+
+/* Seed engine (tuning engine 4): for the first `cap` families (the families
+ * of bgo_plan_family_stats; a PAM family of more than 4,096 guides counts as
+ * several), the number of guide-table entries the last launch verified
+ * against windows, i.e. (window, guide) pairs that share an exact 4-base
+ * block (roofline accounting).  0 for the other engines.  No counterpart in
+ * the reference. */
+int bgo_plan_seed_stats(const bgo_plan* p, int32_t cap, int64_t* verified_out /*[cap]*/);
 
 /* ------------------------------------------------ table assembly on the device
  * The extensive finder's per-hit epilogue and table order

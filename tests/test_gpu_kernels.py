@@ -457,3 +457,125 @@ def test_scan_capped_bounds_records_and_keeps_lowest(gpu, monkeypatch, engine, c
     assert hit_set(full.hits) == want and full.launches == 1
     with pytest.raises(B.RecordLimitExceeded):
         scan(dg, specs, tuning=engine, max_records=1024)
+
+
+# ----------------------------------------------------------- seed engine
+#
+# Engine 4 verifies only the guides that share one of five exact 4-base blocks
+# with a window (a site within 4 mismatches always does).  Its sites must equal
+# the oracle's like every other engine's, at budgets 0..4.
+
+SEED = {"engine": 4}
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_seed_engine_matches_oracle_small(gpu, seed):
+    contigs, guides, max_mm = build_case(seed)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"]) if "N" in "".join(
+        s for _, s in contigs) else DeviceGenome(genome)
+    nm = genome if dg.has_n_mask else {k: v for k, v in genome.items() if not k.startswith("n_mask")}
+    r = scan(dg, [GuideSpec(s, p, max_mm) for s, p in guides], tuning=dict(SEED))
+    assert hit_set(r.hits) == oracle_set(nm, guides, max_mm)
+
+
+def test_seed_engine_families_budgets_ranges_and_replays(gpu):
+    """12 PAM families (several launch groups), per-guide budgets 0..4,
+    20/21-nt guides, position ranges, single strands and graph replays."""
+    rng = np.random.default_rng(3)
+    contigs, guides, _ = build_case(77, length=(20_000, 40_000), n_guides=12)
+    pams = ["NGG", "NRG", "NNGRRT", "NNNRRT", "NG", "TTTV", "SGG", "WGG", "A", "NNNNNN", "BDHV", "GG"]
+    guides = [(s, pams[i]) for i, (s, _) in enumerate(guides)]
+    budgets = [int(rng.integers(0, 5)) for _ in guides]
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
+    want = oracle_set(genome, guides, None, mm_per=budgets)
+    r = scan(dg, specs, tuning=dict(SEED))
+    assert len(r.families) == 12 and hit_set(r.hits) == want
+    L = genome["total_bp"]
+    for (b, e), strands in [((1000, 17_000), 3), ((0, L // 2), 1), ((L // 3, L), 2)]:
+        r = scan(dg, specs, pos_begin=b, pos_end=e, strands=strands, tuning=dict(SEED))
+        assert hit_set(r.hits) == oracle_set(genome, guides, None, mm_per=budgets, pos=(b, e),
+                                             strands=[s for s in (0, 1) if strands & (1 << s)])
+    plan = Plan(dg, specs, hit_capacity=1 << 16, tuning=dict(SEED))
+    for _ in range(3):
+        plan.graph_launch()
+        n, _ = plan.wait()
+        assert hit_set(plan.fetch(n)) == want
+        plan.launch(0, L // 2, 1)   # eager work between replays
+        plan.wait()
+    plan.close()
+
+
+def test_seed_engine_large_family_split_duplicates_and_plants(gpu):
+    """A family of 4,500 guides (two seed tables of at most 4,096 guides, two
+    launches), exact duplicates and near-duplicates of guides (pairs matching
+    in several blocks are recorded once per guide), planted sites at 0..4
+    mismatches on both strands, budgets 0..4, and the capacity re-run."""
+    from brute import IUPAC, plant
+    rng = np.random.default_rng(41)
+    contigs, _, _ = build_case(41, length=(40_000, 60_000), n_guides=1)
+    seqs = ["".join(rng.choice(list("ACGT"), size=20)) for _ in range(4400)]
+    seqs += seqs[:50]                                    # exact duplicates
+    for q in seqs[50:100]:                               # one-base variants
+        i = int(rng.integers(0, 20))
+        seqs.append(q[:i] + "ACGT"[("ACGT".index(q[i]) + 1) % 4] + q[i + 1:])
+    guides = [(q, "NGG") for q in seqs]
+    for gi in rng.choice(len(guides), size=120, replace=False):
+        q, pam = guides[int(gi)]
+        pam_seq = "".join(rng.choice(list(IUPAC[c])) for c in pam)
+        plant(rng, contigs, q, pam_seq, int(rng.integers(0, 5)), "+" if rng.random() < 0.5 else "-")
+    budgets = [int(rng.integers(0, 5)) for _ in guides]
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    specs = [GuideSpec(s, p, k) for (s, p), k in zip(guides, budgets)]
+    want = oracle_set(genome, guides, None, mm_per=budgets)
+    assert len(want) >= 50
+    r = scan(dg, specs, tuning=dict(SEED), hit_capacity=16)
+    assert r.launches == 2 and len(r.families) == 2
+    assert hit_set(r.hits) == want
+    assert r.hits.size == len(want)   # no pair recorded twice
+
+
+def test_seed_engine_verify_counts(gpu):
+    """The verified-entry counts equal, per family, the sum over PAM-passing
+    windows and blocks of the number of the family's guides with the
+    window's 4 bases in that block (counted on the host)."""
+    from beguidedoff_amd.pam import IUPAC_MASK
+    rng = np.random.default_rng(8)
+    contigs, _, _ = build_case(8, length=(30_000, 50_000), n_guides=1)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
+    guides = [("".join(rng.choice(list("ACGT"), size=20)), pam)
+              for pam in ("NGG", "NG", "NNNRRT") for _ in range(300)]
+    specs = [GuideSpec(s, p, 3) for s, p in guides]
+    plan = Plan(dg, specs, hit_capacity=1 << 16, tuning=dict(SEED))
+    plan.launch()
+    plan.wait()
+    got = plan.seed_stats()
+    plan.close()
+    L = genome["total_bp"]
+    for f, pam in enumerate(("NGG", "NG", "NNNRRT")):
+        gcodes = np.stack([encode_dna(s)[:20] for s, p in guides if p == pam]).astype(np.int64)
+        want = 0
+        for key in ("genome_forward", "genome_rc"):
+            c = synth.extract_codes(genome[key], 0, L).astype(np.int64)
+            n = L - 26
+            ok = np.ones(n, bool)
+            for i, ch in enumerate(pam):
+                ok &= ((IUPAC_MASK[ch] >> c[20 + i:20 + i + n]) & 1).astype(bool)
+            idx = np.nonzero(ok)[0]
+            for j in range(5):
+                wk = sum(c[idx + 4 * j + b] << (2 * b) for b in range(4))
+                gk = sum(gcodes[:, 4 * j + b] << (2 * b) for b in range(4))
+                want += int(np.bincount(gk, minlength=256)[wk].sum())
+        assert got[f] == want, (pam, got[f], want)
+
+
+def test_seed_engine_needs_budgets_up_to_4(gpu):
+    contigs, guides, _ = build_case(2)
+    genome = synth.genome_from_strings(contigs)
+    dg = DeviceGenome(genome)
+    with pytest.raises(RuntimeError, match="budgets <= 4"):
+        scan(dg, [GuideSpec(s, p, 5) for s, p in guides], tuning=dict(SEED))
