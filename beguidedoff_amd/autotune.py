@@ -1,6 +1,7 @@
 """Autotuner of the batched kernel's launch shape.
 
-Tunables (bgo_tuning): compare engine (VALU xor/popcount, int8 or fp4 MFMA),
+Tunables (bgo_tuning): compare engine (VALU xor/popcount, int8 or fp4 MFMA,
+seed-and-verify when every budget is <= 4),
 positions per work item (genome tile length), waves per block, resident
 blocks per CU, and strand interleave of the work items.
 Each candidate runs the real kernel on a slice of the resident genome with
@@ -21,7 +22,7 @@ from . import _lib
 from .batched import DeviceGenome, GuideSpec, Plan
 
 DEFAULT_SPACE = {
-    "engine": [1, 2, 3],
+    "engine": [1, 2, 3, 4],
     "chunk_positions": [32768, 131072],
     "waves_per_block": [2, 4],
     "blocks_per_cu": [0],
@@ -73,7 +74,9 @@ def autotune(genome: DeviceGenome, specs: Sequence[GuideSpec], sample_positions:
         hit = _load_cache().get(key)
         if hit:
             return dict(hit["tuning"]), []
-    space = space or DEFAULT_SPACE
+    space = dict(space or DEFAULT_SPACE)
+    if "engine" in space and max((s.max_mm for s in specs), default=0) > 4:
+        space["engine"] = [e for e in space["engine"] if e != 4]  # seed: budgets <= 4 only
     names = list(space)
     L = genome.total_bp
     begin = max(0, L // 2 - sample_positions // 2)

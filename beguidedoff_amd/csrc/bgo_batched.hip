@@ -729,9 +729,10 @@ __host__ __device__ inline uint32_t seed_other(uint32_t w, uint32_t j) {
   return (w & ((1u << (4 * j)) - 1u)) | ((w >> (4 * j + 4)) << (4 * j));
 }
 
-// Exact path of a seed candidate: the guide behind table word widx, its full
-// 2-bit count, and the pair only from the first block in which it matches
-// exactly (so a pair matching in several blocks is recorded once).
+// Exact path of a seed candidate: the guide behind table word widx (an entry
+// of the window's key in block j), its full 2-bit count, and the pair only if
+// block j is the first block in which it matches exactly, so a pair matching
+// in several blocks is recorded once.
 __device__ inline void seed_exact(const Args& a, const Tab& t, uint32_t s, uint32_t lane,
                                   uint32_t W0, uint32_t W1, uint32_t pos, uint32_t j,
                                   uint32_t widx, bool cand) {
@@ -742,9 +743,10 @@ __device__ inline void seed_exact(const Args& a, const Tab& t, uint32_t s, uint3
     const uint32_t m = ((W0 ^ t.g01[2 * gi]) | (W1 ^ t.g01[2 * gi + 1])) & kWin;
     uint32_t nz = m | (m >> 1);
     nz |= nz >> 2;  // bit 4b: block b has a mismatch
-    const uint32_t below = 0x11111u & ((1u << (4 * j)) - 1u);
+    const uint32_t upto = 0x11111u & ((1u << (4 * j + 1)) - 1u);  // blocks 0..j
+    const uint32_t below = 0x11111u & ((1u << (4 * j)) - 1u);     // blocks 0..j-1
     c = static_cast<uint32_t>(__popc(m));
-    rec = (nz & below) == below && static_cast<int>(c) <= t.gk[gi];
+    rec = (nz & upto) == below && static_cast<int>(c) <= t.gk[gi];
   }
   if (ballot(rec)) record_candidates(a, t, s, gi, pos, W0, W1, c, rec, lane);
 }
@@ -771,29 +773,27 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, u
     const uint32_t* sj = st + j * (kSeedKeys + 1);
     const uint32_t beg = sj[key];
     const uint32_t end = has ? sj[key + 1] : beg;
-    const uint32_t* ej = en + j * n_g;
     verified += end - beg;
-    // Four entries per step, read together (each family's entries end with
-    // kSeedPad spare words, so the reads past a lane's list stay in the
-    // table; their results are masked).
+    // Four entries per step, read together.  A lane whose key's entries are
+    // done goes on reading the following ones while the wave's longest walk
+    // lasts; only the rare candidates are checked to lie in the lane's own
+    // key (i + q < end), and the reads stay in the family's table (clamped to
+    // its last entries, which end with kSeedPad spare words; a clamped read
+    // is never in the lane's key).
+    const uint32_t last = kSeedBlocks * n_g - 1;
     for (uint32_t i = beg; ballot(i < end); i += 4) {
-      uint32_t x[4];
-#pragma unroll
-      for (uint32_t q = 0; q < 4; ++q) x[q] = ej[i + q];
-      uint32_t hit = 0;  // bit q: entry i + q is a candidate
+      const uint32_t b = min(j * n_g + i, last);
+      uint32_t p[4];
 #pragma unroll
       for (uint32_t q = 0; q < 4; ++q) {
-        const uint32_t d = x[q] ^ wo;
-        const uint32_t m = (d | (d >> 16)) & 0xFFFFu;
-        const uint32_t ok = static_cast<uint32_t>(i + q < end) &
-                            static_cast<uint32_t>(static_cast<uint32_t>(__popc(m)) <= kmax);
-        hit |= ok << q;
+        const uint32_t d = en[b + q] ^ wo;
+        p[q] = static_cast<uint32_t>(__popc((d | (d >> 16)) & 0xFFFFu));
       }
-      if (ballot(hit != 0)) {
+      if (ballot(min(min(p[0], p[1]), min(p[2], p[3])) <= kmax)) {
 #pragma nounroll
         for (uint32_t q = 0; q < 4; ++q) {
-          const bool cand = (hit >> q) & 1u;
-          if (ballot(cand)) seed_exact(a, t, s, lane, W0, W1, pos, j, F.seed_ent + j * n_g + i + q, cand);
+          const bool cand = p[q] <= kmax && i + q < end;
+          if (ballot(cand)) seed_exact(a, t, s, lane, W0, W1, pos, j, F.seed_ent + b + q, cand);
         }
       }
     }
@@ -1504,6 +1504,28 @@ size_t seed_block_lds(size_t image_bytes, int nf) {
   return (image_bytes + 255) / 256 * 256 + bk::wave_lds_bytes(nf, 4) * (bk::kSeedBlock / 64);
 }
 
+// First family of each seed-engine launch group, for family parts of these
+// guide counts in order: greedily, as many parts (at most kSeedMaxFam) as
+// fit one block's LDS with their tables.
+std::vector<int> seed_groups(const std::vector<uint32_t>& part_guides) {
+  std::vector<int> first;
+  const int n = static_cast<int>(part_guides.size());
+  int f = 0;
+  while (f < n) {
+    first.push_back(f);
+    size_t img = seed_image_bytes(part_guides[f]);
+    int nf = 1;
+    while (f + nf < n && nf < bk::kSeedMaxFam) {
+      const size_t more = img + seed_image_bytes(part_guides[f + nf]);
+      if (seed_block_lds(more, nf + 1) > kSeedLdsBytes) break;
+      img = more;
+      ++nf;
+    }
+    f += nf;
+  }
+  return first;
+}
+
 // Per launch group: the image copied to LDS (per family, then per block j of
 // the 20 compared bases: 257 key starts, then one word per guide sorted by
 // the 8-bit key of its bases 4j..4j+3: the other 16 bases, b0 plane bits low
@@ -1578,17 +1600,27 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     // The engine with the lower modelled cost.  Per family, in ms per unit of
     // PAM pass rate on a 3.09 Gbp genome (fitted to single-family and mixed
     // launches on MI355X): VALU in place 0.86 + 0.94 G, VALU through the
-    // window buffer 6.7 + 3.7 ceil(G / 8), fp4 8.1 + 4.0 ceil(G / 32).  The
-    // streaming both engines share drops out.
-    double valu = 0.0, fp4 = 0.0;
+    // window buffer 6.7 + 3.7 ceil(G / 8), fp4 8.1 + 4.0 ceil(G / 32), seed
+    // (budgets <= 4 only) 36 + 27 G / 1024, plus 0.9 ms for each launch the
+    // seed tables need beyond the first (a launch streams the genome once).
+    // The streaming all engines share drops out.
+    double valu = 0.0, fp4 = 0.0, seed = 0.0;
+    bool seed_ok = true;
+    std::vector<uint32_t> parts;
     for (int f = 0; f < n_fam; ++f) {
       const double pass = pam_pass_rate(proto[f].nibs);
       const int g = static_cast<int>(members[f].size());
       valu += pass * (g <= direct_limit(P, pass) ? 0.86 + 0.94 * g : 6.7 + 3.7 * ((g + 7) / 8));
       fp4 += pass * (8.1 + 4.0 * ((g + 31) / 32));
+      seed += pass * (36.0 + 27.0 * g / 1024.0);
+      for (int32_t i : members[f]) seed_ok = seed_ok && guides[i].max_mm <= 4;
+      for (size_t b = 0; b < static_cast<size_t>(g); b += kSeedMaxGuides)
+        parts.push_back(static_cast<uint32_t>(std::min<size_t>(g - b, kSeedMaxGuides)));
     }
+    seed += 0.9 * (static_cast<double>(seed_groups(parts).size()) - 1.0);
     P->tune.engine = fp4 < valu ? 3 : 1;
-    P->tune.guide_unroll = P->tune.engine >= 2 ? 32 : bk::kUnroll;
+    if (seed_ok && seed < std::min(fp4, valu)) P->tune.engine = 4;
+    P->tune.guide_unroll = P->tune.engine == 4 ? 1 : P->tune.engine >= 2 ? 32 : bk::kUnroll;
   }
   if (P->tune.engine == 4) {
     // The seed engine's pigeonhole (five exact 4-base blocks) covers budgets
@@ -1667,19 +1699,9 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
   if (P->tune.engine == 4) {
     // Launch groups of families whose seed tables and wave buffers fit one
     // block's LDS together.
-    int f = 0;
-    while (f < n_fam_parts) {
-      P->group_first.push_back(f);
-      size_t img = seed_image_bytes(P->fams[f].g_live - P->fams[f].g_begin);
-      int nf = 1;
-      while (f + nf < n_fam_parts && nf < bk::kSeedMaxFam) {
-        const size_t more = img + seed_image_bytes(P->fams[f + nf].g_live - P->fams[f + nf].g_begin);
-        if (seed_block_lds(more, nf + 1) > kSeedLdsBytes) break;
-        img = more;
-        ++nf;
-      }
-      f += nf;
-    }
+    std::vector<uint32_t> sizes;
+    for (const Fam& F : P->fams) sizes.push_back(F.g_live - F.g_begin);
+    P->group_first = seed_groups(sizes);
   } else {
     for (int f = 0; f < n_fam_parts; f += bk::kMaxFam) P->group_first.push_back(f);
   }

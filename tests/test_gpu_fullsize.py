@@ -11,8 +11,8 @@ combine:
   batched engine;
 * oracle rescans of position slices (genome start and end, N-block edges,
   every second contig junction) match the sites of the full scan in them;
-* with 8192 guides the VALU, int8 and fp4 engines agree, and an oracle slice
-  matches;
+* with 8192 guides the VALU, int8, fp4 and seed engines agree, and an oracle
+  slice matches;
 * serving on the resident genome: Poisson arrivals of config-3 requests, each
   request's records equal a scan of it alone and the oracle on a slice.
 """
@@ -131,8 +131,8 @@ def test_budget_threshold_is_exact(mixed, mixed_5mm):
 def test_engines_agree_at_full_size(mixed):
     w, dg = mixed
     specs = specs_of(w.guides, 4)
-    sets = [packed(scan(dg, specs, tuning={"engine": e}).hits) for e in (1, 2, 3)]
-    assert np.array_equal(sets[0], sets[1]) and np.array_equal(sets[0], sets[2])
+    sets = [packed(scan(dg, specs, tuning={"engine": e}).hits) for e in (1, 2, 3, 4)]
+    assert all(np.array_equal(sets[0], x) for x in sets[1:])
     assert sets[0].size > 100_000
 
 
@@ -225,15 +225,20 @@ def test_config3_5mm_plants_found(gpu):
 
 def test_8192_guides_engines_agree(mixed):
     """Config 4: 8192 NGG guides at <= 4 mm; the VALU compare, the int8
-    one-hot MFMA contraction and fp4 give the same sites, which equal the
-    oracle on a slice across a contig junction."""
+    one-hot MFMA contraction, fp4 and the seed engine (two 4,096-guide
+    tables) give the same sites, which equal the oracle on a slice across a
+    contig junction; the cost model picks the seed engine."""
     w, dg = mixed
     g = w.genome
     seqs = synth.random_guides(np.random.default_rng(0), 8192)
     specs = [GuideSpec(s, "NGG", 4) for s in seqs]
-    res = [scan(dg, specs, tuning={"engine": e}).hits for e in (1, 2, 3)]
+    res = [scan(dg, specs, tuning={"engine": e}).hits for e in (1, 2, 3, 4)]
     sets = [packed(h) for h in res]
-    assert np.array_equal(sets[0], sets[1]) and np.array_equal(sets[0], sets[2])
+    assert all(np.array_equal(sets[0], x) for x in sets[1:])
+    from beguidedoff_amd.batched import Plan
+    p = Plan(dg, specs, hit_capacity=1 << 10)
+    assert p.tuning()["engine"] == 4
+    p.close()
     assert sets[0].size > 500_000
     b = int(g["chrom_offsets"][5]) - 250_000
     e = b + 500_000

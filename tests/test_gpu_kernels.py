@@ -362,14 +362,16 @@ def test_direct_compare_every_guide_count(gpu):
 
 def test_auto_engine_choice(gpu):
     """Engine 0 takes the cheaper engine by the per-family cost model: the VALU
-    compare for small sets (in-place families) and fp4 for large families."""
+    compare for small sets (in-place families), fp4 for larger families, and
+    the seed engine for families of thousands of guides when every budget is
+    at most 4."""
     contigs, _, _ = build_case(3, length=(5_000, 8_000), n_guides=1)
     genome = synth.genome_from_strings(contigs)
     dg = DeviceGenome(genome, genome["n_mask_le_fwd"], genome["n_mask_le_rc"])
     rng = np.random.default_rng(2)
 
-    def engine_for(counts):
-        specs = [GuideSpec("".join(rng.choice(list("ACGT"), size=20)), pam, 4)
+    def engine_for(counts, budget=4):
+        specs = [GuideSpec("".join(rng.choice(list("ACGT"), size=20)), pam, budget)
                  for pam, n in counts.items() for _ in range(n)]
         p = Plan(dg, specs, hit_capacity=1 << 12)
         try:
@@ -381,6 +383,9 @@ def test_auto_engine_choice(gpu):
     assert engine_for({"NGG": 36, "NNNRRT": 21, "NNGRRT": 16, "NG": 7}) == 1
     assert engine_for({"NGG": 64, "NNNRRT": 64, "NNGRRT": 64, "NG": 64}) == 3
     assert engine_for({"NGG": 512, "NNNRRT": 256, "NNGRRT": 128, "NG": 128}) == 3
+    assert engine_for({"NGG": 2048}) == 4
+    assert engine_for({"NG": 1024, "NNNRRT": 1024}) == 4
+    assert engine_for({"NGG": 2048}, budget=5) == 3
 
 
 def block_genome(seed=3, blocks=24, block_len=1500):
