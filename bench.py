@@ -98,6 +98,11 @@ TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_602_305_472,
 # the traffic.
 UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.696, "mfma": 0.502}}
 
+# The seed engine on 8192 NGG guides (two 4,096-guide tables, two launches
+# per step): VALU busy from SQ_INSTS_VALU over GRBM_GUI_ACTIVE SIMD-cycles.
+SEED_BUSY = {("hg38_8k", 4, 4): ({"valu": 0.894},
+                                 "profiles/seed_scan_kernel_hg38_8k_4mm.md")}
+
 
 def site_keys(guide, strand, pos, n_mm) -> np.ndarray:
     """(guide, strand, position, N-corrected mismatches) of each site packed
@@ -120,7 +125,8 @@ def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, f
            "frac": round(achieved / peak, 4),
            "traffic": traffic[0] if traffic else None,
            "traffic_source": traffic[1] if traffic else None,
-           "kernel": "batched_scan_kernel", "ops_per_launch": int(alg),
+           "kernel": "seed_scan_kernel" if engine == 4 else "batched_scan_kernel",
+           "ops_per_launch": int(alg),
            "pairs_per_launch": pairs,
            "survey_ops_per_launch": int(survey_ops),
            "hbm_gbps_planes": round(2 * L / 4 / secs / 1e9, 1)}
@@ -131,6 +137,13 @@ def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, f
             "are chains of dependent VALU/LDS steps at 4 waves per SIMD, and the MFMA pipe "
             "idles while a wave runs them" if busy[top] < 0.8 else f"{top.upper()}")
         out["unit_busy"] = dict(busy, source=traffic[1] if traffic else None)
+    seed_busy = SEED_BUSY.get((workload, engine, max_mm))
+    if engine == 4 and seed_busy:
+        out["binding_unit"] = (
+            f"VALU {seed_busy[0]['valu']:.0%} busy; the counted lane-ops are 56% of the issued "
+            "VALU lane-slots: a wave walks its 64 windows' guide lists for as long as the "
+            "longest of them")
+        out["unit_busy"] = dict(seed_busy[0], source=seed_busy[1])
     # The survey's per-unit models of the same launch: 11 INT32 VALU ops per
     # (guide, window) pair + 3 per window and family, against the INT32 VALU
     # peak; 160 int8 ops per pair against the dense int8 MFMA peak.  The
