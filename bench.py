@@ -693,6 +693,10 @@ def main() -> int:
     for eng, name in () if args.no_engines else legs:
         t_eng = dict(plan.tuning())
         t_eng["engine"] = eng
+        if eng == 4 or engine == 4:
+            # the seed engine runs one 16-wave block per CU; the others keep the
+            # library's occupancy and block shape
+            t_eng.update(blocks_per_cu=0, waves_per_block=4)
         p2 = Plan(dg, specs, tuning=t_eng, hit_capacity=default_capacity(len(specs)))
         times = []
         for _ in range(3):

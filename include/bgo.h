@@ -158,7 +158,7 @@ typedef struct bgo_hit {
  * autotuned value). */
 typedef struct bgo_tuning {
   int32_t chunk_positions;   /* positions per work item (multiple of 64) */
-  int32_t waves_per_block;   /* 1, 2 or 4 */
+  int32_t waves_per_block;   /* 1, 2 or 4 (engine 4 runs one 16-wave block per CU) */
   int32_t blocks_per_cu;     /* resident blocks requested per CU */
   int32_t engine;            /* 0 auto (VALU, fp4 MFMA or seed, whichever a
                                 per-family cost model of the guide counts and
