@@ -257,3 +257,19 @@ def test_device_assembly_names_out_of_order(gpu):
     pd.testing.assert_frame_equal(df, host.search(guides=inputs, return_all=True, verbose=False))
     z = df[df["score"] == 0]
     assert list(zip(z["ot_chromosome"], z["ot_position"]))[:13] == zero
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_seed_engine_tables_match_oracle(gpu, seed):
+    """The finder's table with the seed-and-verify compare (engine 4, budgets
+    <= 4) equals the oracle finder's, assembled on the device and on the
+    host."""
+    genome, guides = assembly_case(seed)
+    kw = {"return_all": True, "count_score_threshold": 5.0}
+    want = FinderOracle(genome, max_mismatches=4).search(guides, **kw)
+    for device_assembly in (True, False):
+        f = finder(genome, "batched", max_mismatches=4, tuning={"engine": 4})
+        f.device_assembly = device_assembly
+        got = f.search(guides=guides, verbose=False, **kw)
+        pd.testing.assert_frame_equal(got, want)
+    assert len(want) > 5
