@@ -98,6 +98,8 @@ struct Tab {
   const int4* __restrict__ btab;         // MFMA engines: guide operand tiles
   bgo_hit* rec_stage;                    // MFMA and seed engines: the wave's staged records (LDS), else null
   uint32_t* rec_cnt;                     // records staged (LDS)
+  const uint32_t* runs;                  // [2][2] (LDS): position base and strand of the
+                                         // runs whose windows the buffers hold, by entry flag
   const uint32_t* stab;                  // seed engine: the block's seed table (LDS)
   const uint32_t* __restrict__ sgix;     // seed engine: guide of each table word
 };
@@ -111,19 +113,31 @@ __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p
 // holds one batch plus one append pass (64 windows); after a batch the at
 // most 64 windows behind it move to the front, so indices never wrap.  A slot
 // is {b0 window bits | offset bits 0..11 << 20, b1 window bits | offset bits
-// 12..23 << 20}, the offset counted from the run's 32-aligned start.
+// 12..22 << 20 | run flag << 31}, the offset counted from the 32-aligned start
+// of the run (work item) that appended it.  A wave carries the windows left in
+// its buffers at the end of a run into its next run, so the entries of two
+// consecutive runs can share a batch: the flag (the parity of the wave's run
+// count) picks the run's position base and strand from Tab::runs.
 __host__ __device__ constexpr uint32_t ring_flush(int engine) { return engine == 3 ? 128 : 64; }
 __host__ __device__ constexpr uint32_t ring_size(int engine) { return ring_flush(engine) + 64; }
 
 // Bytes of LDS one wave uses with nf families (see the kernel's layout).
 __host__ __device__ constexpr size_t wave_lds_bytes(int nf, int engine) {
   return static_cast<size_t>(nf) * ring_size(engine) * 8 + kMaxFam * 8 +
-         static_cast<size_t>(nf) * 64 * 4 + kMaxFam * 4 + 64 * 8 +
+         static_cast<size_t>(nf) * 64 * 4 + kMaxFam * 4 + 16 + 64 * 8 +
          (engine >= 2 ? kRecStage * sizeof(bgo_hit) + 16 : 0) +
          (engine == 4 ? static_cast<size_t>(nf) * 64 * 4 : 0);
 }
 
-__device__ inline uint32_t slot_pos(uint2 e) { return (e.x >> 20) | ((e.y >> 20) << 12); }
+__device__ inline uint32_t slot_pos(uint2 e) { return (e.x >> 20) | (((e.y >> 20) & 0x7FFu) << 12); }
+
+// Position and strand of a buffered window.
+__device__ inline uint32_t entry_pos(const uint32_t* runs, uint2 e) {
+  return runs[2 * (e.y >> 31)] + slot_pos(e);
+}
+__device__ inline uint32_t entry_strand(const uint32_t* runs, uint2 e) {
+  return runs[2 * (e.y >> 31) + 1];
+}
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -230,12 +244,12 @@ __device__ inline void record_candidates(const Args& a, const Tab& t, uint32_t s
 
 // Compare the first n (<= 64) windows of a family ring with all guides of
 // the family.  One window per lane; guides are wave-uniform.
-__device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, uint32_t s,
-                                    const uint2* L, uint32_t n, uint32_t cs, uint32_t lane) {
+__device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, const uint2* L,
+                                    uint32_t n, uint32_t lane) {
   if (lane >= n) return;
   const uint2 e = L[lane];
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
-  const uint32_t pos = cs + slot_pos(e);
+  const uint32_t pos = entry_pos(t.runs, e), s = entry_strand(t.runs, e);
   const uint32_t kmax = F.kmax;
   const uint32_t* __restrict__ g01 = t.g01;
   for (uint32_t g = F.g_begin; g < F.g_end; g += kUnroll) {
@@ -401,13 +415,12 @@ __device__ inline uint32_t fp4_dword(uint32_t c, uint32_t d, uint32_t pad) {
 // accumulator is non-negative: bit r (16 + r) of `bits` = register r of the
 // product with A tile abase (abase + 1).
 template <int ENGINE>
-__device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                  uint32_t n, uint32_t cs, uint32_t lane,
-                                  uint32_t bits, uint32_t gt, uint32_t abase) {
+__device__ inline void mfma_exact(const Args& a, const Tab& t, const uint2* L, uint32_t n,
+                                  uint32_t lane, uint32_t bits, uint32_t gt, uint32_t abase) {
   const uint32_t m = lane & 31u, g = lane >> 5;
   while (ballot(bits != 0)) {
     bool cand = false;
-    uint32_t gi = 0, pos = 0, W0 = 0, W1 = 0, c = 0;
+    uint32_t gi = 0, pos = 0, s = 0, W0 = 0, W1 = 0, c = 0;
     if (bits) {
       const uint32_t bit = __builtin_ctz(bits);
       bits &= bits - 1;
@@ -419,7 +432,8 @@ __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const
         const uint2 e = L[widx];
         W0 = e.x & kWin;
         W1 = e.y & kWin;
-        pos = cs + slot_pos(e);
+        pos = entry_pos(t.runs, e);
+        s = entry_strand(t.runs, e);
         c = __popc((W0 ^ t.g01[2 * gi]) | (W1 ^ t.g01[2 * gi + 1]));
         cand = static_cast<int>(c) <= t.gk[gi];
       }
@@ -436,9 +450,9 @@ __device__ inline void mfma_exact(const Args& a, const Tab& t, uint32_t s, const
 // (as 32-bit patterns): the common case returns after one AND-reduction and
 // one ballot.
 template <int ENGINE>
-__device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                  uint32_t n, uint32_t cs, uint32_t lane,
-                                  const v16i& r0, const v16i& r1, uint32_t gt, uint32_t abase) {
+__device__ inline void mfma_signs(const Args& a, const Tab& t, const uint2* L, uint32_t n,
+                                  uint32_t lane, const v16i& r0, const v16i& r1, uint32_t gt,
+                                  uint32_t abase) {
   uint32_t all = static_cast<uint32_t>(r0[0]);
 #pragma unroll
   for (int r = 1; r < 16; ++r) all &= static_cast<uint32_t>(r0[r]);
@@ -454,12 +468,12 @@ __device__ inline void mfma_signs(const Args& a, const Tab& t, uint32_t s, const
   for (int r = 15; r >= 0; --r) neg = (neg << 1) | (static_cast<uint32_t>(r1[r]) >> 31);
 #pragma unroll
   for (int r = 15; r >= 0; --r) neg = (neg << 1) | (static_cast<uint32_t>(r0[r]) >> 31);
-  mfma_exact<ENGINE>(a, t, s, L, n, cs, lane, ~neg, gt, abase);
+  mfma_exact<ENGINE>(a, t, L, n, lane, ~neg, gt, abase);
 }
 
 // int8: one 32-guide tile against the (up to) two 32-window A tiles.
-__device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                               uint32_t n, uint32_t cs, uint32_t lane, bool two,
+__device__ inline void i8_tile(const Args& a, const Tab& t, const uint2* L, uint32_t n,
+                               uint32_t lane, bool two,
                                const v4i (&A)[2][2], int4 b0, int4 b1, uint32_t gt) {
   const v16i zero = {};
   const v4i B0 = {b0.x, b0.y, b0.z, b0.w}, B1 = {b1.x, b1.y, b1.z, b1.w};
@@ -470,7 +484,7 @@ __device__ inline void i8_tile(const Args& a, const Tab& t, uint32_t s, const ui
     acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][0], B0, zero, 0, 0, 0);
     acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[1][1], B1, acc1, 0, 0, 0);
   }
-  mfma_signs<2>(a, t, s, L, n, cs, lane, acc0, acc1, gt, 0);
+  mfma_signs<2>(a, t, L, n, lane, acc0, acc1, gt, 0);
 }
 
 // E8M0 scales of the fp4 products, one per byte of a single register (the
@@ -526,18 +540,18 @@ __device__ inline uint32_t and16(const v16i& p) {
 }
 
 // fp4: one 32-guide tile against the nat (1..4) 32-window A tiles in use.
-__device__ inline void fp4_tile(const Args& a, const Tab& t, uint32_t s, const uint2* L,
-                                uint32_t n, uint32_t cs, uint32_t lane,
+__device__ inline void fp4_tile(const Args& a, const Tab& t, const uint2* L, uint32_t n,
+                                uint32_t lane,
                                 uint32_t nat, const v8i (&A)[4], v4i b, uint32_t gt) {
   v16i p0 = fp4_mma(A[0], b);
   v16i p1 = v16i{} - 1;
   if (nat > 1) p1 = fp4_mma(A[1], b);
-  mfma_signs<3>(a, t, s, L, n, cs, lane, p0, p1, gt, 0);
+  mfma_signs<3>(a, t, L, n, lane, p0, p1, gt, 0);
   if (nat > 2) {
     p0 = fp4_mma(A[2], b);
     p1 = v16i{} - 1;
     if (nat > 3) p1 = fp4_mma(A[3], b);
-    mfma_signs<3>(a, t, s, L, n, cs, lane, p0, p1, gt, 2);
+    mfma_signs<3>(a, t, L, n, lane, p0, p1, gt, 2);
   }
 }
 
@@ -571,9 +585,8 @@ __device__ inline bool leftover_chains(const v8i (&A)[4], v4i b0, v4i b1, const 
 // with spare all-zero tiles, so the look-ahead loads of later tiles stay in
 // bounds.
 template <int ENGINE>
-__device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam& F, uint32_t s,
-                                         const uint2* L, uint32_t n, uint32_t cs,
-                                         uint32_t lane) {
+__device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam& F,
+                                         const uint2* L, uint32_t n, uint32_t lane) {
   constexpr uint32_t kAT = ring_flush(ENGINE) / 32;  // A tiles
   const uint32_t m = lane & 31u, g = lane >> 5;
   if constexpr (ENGINE == 3) {
@@ -656,30 +669,30 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
         if (ballot((any & kField3Mask) != 0)) {
           // the rare triple is redone from reloaded operands (the registers
           // already hold the next triple)
-          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, btile(ti), F.g_begin + 32 * ti);
-          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, btile(ti + 1), F.g_begin + 32 * ti + 32);
-          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, btile(ti + 2), F.g_begin + 32 * ti + 64);
+          fp4_tile(a, t, L, n, lane, kAT, A, btile(ti), F.g_begin + 32 * ti);
+          fp4_tile(a, t, L, n, lane, kAT, A, btile(ti + 1), F.g_begin + 32 * ti + 32);
+          fp4_tile(a, t, L, n, lane, kAT, A, btile(ti + 2), F.g_begin + 32 * ti + 64);
         }
       }
       // the last one or two guide tiles (b0, b1 hold tiles ti, ti + 1)
       const uint32_t left = n_tiles - ti;
       if (left == 1) {
         if (leftover_chains<kAT, kAT>(A, b0, b0, k3))
-          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, b0, F.g_begin + 32 * ti);
+          fp4_tile(a, t, L, n, lane, kAT, A, b0, F.g_begin + 32 * ti);
       } else if (left == 2) {
         if (leftover_chains<kAT, 2 * kAT>(A, b0, b1, k3)) {
-          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, b0, F.g_begin + 32 * ti);
-          fp4_tile(a, t, s, L, n, cs, lane, kAT, A, b1, F.g_begin + 32 * ti + 32);
+          fp4_tile(a, t, L, n, lane, kAT, A, b0, F.g_begin + 32 * ti);
+          fp4_tile(a, t, L, n, lane, kAT, A, b1, F.g_begin + 32 * ti + 32);
         }
       }
       __builtin_amdgcn_s_setprio(1);
       return;
     }
     for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
-      fp4_tile(a, t, s, L, n, cs, lane, nat, A, b0, F.g_begin + 32 * ti);
+      fp4_tile(a, t, L, n, lane, nat, A, b0, F.g_begin + 32 * ti);
       b0 = btile(ti + 2);
       if (ti + 1 < n_tiles)
-        fp4_tile(a, t, s, L, n, cs, lane, nat, A, b1, F.g_begin + 32 * ti + 32);
+        fp4_tile(a, t, L, n, lane, nat, A, b1, F.g_begin + 32 * ti + 32);
       b1 = btile(ti + 3);
     }
   } else {
@@ -706,7 +719,7 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
       const int4 c0 = b0, c1 = b1;
       b0 = bp[128 + lane];
       b1 = bp[192 + lane];
-      i8_tile(a, t, s, L, n, cs, lane, two, A, c0, c1, gt);
+      i8_tile(a, t, L, n, lane, two, A, c0, c1, gt);
     }
   }
 }
@@ -754,14 +767,13 @@ __device__ inline void seed_exact(const Args& a, const Tab& t, uint32_t s, uint3
 // Compare the first n (<= 64) windows of a family ring with the family's
 // guides through the seed table: one window per lane; per block, each lane
 // walks its own key's guides, the wave as long as its longest walk.
-__device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, uint32_t s,
-                                    const uint2* L, uint32_t n, uint32_t cs, uint32_t lane,
-                                    uint32_t* vcnt) {
+__device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, const uint2* L,
+                                    uint32_t n, uint32_t lane, uint32_t* vcnt) {
   uint32_t verified = 0;
   const bool has = lane < n;
   const uint2 e = has ? L[lane] : make_uint2(0u, 0u);
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
-  const uint32_t pos = cs + slot_pos(e);
+  const uint32_t pos = entry_pos(t.runs, e), s = entry_strand(t.runs, e);
   const uint32_t kmax = F.kmax;
   const uint32_t n_g = F.g_live - F.g_begin;
   const uint32_t* st = t.stab + F.seed_start;
@@ -805,7 +817,8 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, u
 struct WaveState {
   uint2* lists;                  // [NF][ring_size] window buffers
   uint32_t* pws;                 // [NF][64] pass words of the current blocks
-  uint32_t* counts;              // [kMaxFam] windows in each buffer
+  uint32_t* counts;              // [kMaxFam] windows in each buffer (carried-over ones << 16)
+  uint32_t* runs;                // [2][2] position base and strand of the runs by entry flag
   unsigned long long* wstats;    // [kMaxFam] windows compared
   uint2* stage;                  // [64] plane words of the next step's blocks
   uint32_t* vcnt;                // seed engine: [NF][64] table entries each lane verified
@@ -819,22 +832,22 @@ struct WaveState {
 // and move the at most 64 windows behind it to the front.
 template <int ENGINE>
 __device__ inline void flush_front(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
-                                   uint32_t s, uint2* L, uint32_t cb, uint32_t lane) {
+                                   uint2* L, uint32_t lane) {
   constexpr uint32_t kFlush = ring_flush(ENGINE);
   wave_sync();
   if constexpr (ENGINE == 2 || ENGINE == 3)
-    compare_list_mfma<ENGINE>(a, t, t.fam[f], s, L, kFlush, cb, lane);
+    compare_list_mfma<ENGINE>(a, t, t.fam[f], L, kFlush, lane);
   else if constexpr (ENGINE == 4)
   {
     __builtin_amdgcn_s_setprio(0);
-    seed_compare(a, t, t.fam[f], s, L, kFlush, cb, lane, w.vcnt + f * 64);
+    seed_compare(a, t, t.fam[f], L, kFlush, lane, w.vcnt + f * 64);
     __builtin_amdgcn_s_setprio(1);
   }
   else
   {
     // VALU batch compare at wave priority 0, like the fp4 full-batch compare
     __builtin_amdgcn_s_setprio(0);
-    compare_list(a, t, t.fam[f], s, L, kFlush, cb, lane);
+    compare_list(a, t, t.fam[f], L, kFlush, lane);
     __builtin_amdgcn_s_setprio(1);
   }
   const uint2 e = L[kFlush + lane];  // LDS operations of a wave complete in order
@@ -850,24 +863,28 @@ __device__ inline void flush_front(const Args& a, const Tab& t, const WaveState&
 // room for one such step.
 template <int ENGINE>
 __device__ inline void append_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
-                                     uint32_t s, uint32_t pw, uint32_t bb, uint2 qa, uint2 qb,
-                                     uint32_t cb, uint32_t lane) {
+                                     uint32_t pw, uint32_t bb, uint2 qa, uint2 qb, uint32_t cb,
+                                     uint32_t flag, uint32_t lane) {
   constexpr uint32_t kSize = ring_size(ENGINE), kFlush = ring_flush(ENGINE);
   uint2* L = w.ring<ENGINE>(f);
-  uint32_t cnt = uniform(w.counts[f]);
+  // counts[f]: windows in the buffer (low half), of which the first (high
+  // half) were carried over from the wave's previous run
+  const uint32_t cw = uniform(w.counts[f]);
+  uint32_t cnt = cw & 0xFFFFu, old = cw >> 16;
   // Window offsets are from the 32-aligned base cb, so the block's offset
   // is a multiple of 32 and the entry's offset fields are block constants
   // plus the bit index.
   const uint32_t off0 = (bb << 5) - cb;
-  const uint32_t x0 = off0 << 20, y0 = (off0 >> 12) << 20;
+  const uint32_t x0 = off0 << 20, y0 = ((off0 >> 12) << 20) | (flag << 31);
   // Each pass appends the next window of every lane that has one; passes run
   // back to back while the buffer has room for a full pass (64 slots), and a
   // batch is compared when it has not.
   uint64_t act = ballot(pw != 0);
   while (act) {
     if (cnt > kSize - 64) {
-      flush_front<ENGINE>(a, t, w, f, s, L, cb, lane);
+      flush_front<ENGINE>(a, t, w, f, L, lane);
       cnt -= kFlush;
+      old = old > kFlush ? old - kFlush : 0u;
       continue;
     }
     do {
@@ -884,7 +901,7 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
       act = ballot(pw != 0);
     } while (act && cnt <= kSize - 64);
   }
-  if (lane == 0) w.counts[f] = cnt;
+  if (lane == 0) w.counts[f] = cnt | (old << 16);
   wave_sync();
 }
 
@@ -893,19 +910,57 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
 // per PAM position for 32 positions), then the passing windows are appended
 // to the family's ring in LDS; every ring_flush(ENGINE) buffered windows are
 // compared against the family's guides.
+// Compare every window in family f's buffer and empty it.
+template <int ENGINE>
+__device__ inline void drain_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
+                                    uint32_t lane) {
+  uint32_t c = uniform(w.counts[f]) & 0xFFFFu;
+  if (c > ring_flush(ENGINE)) {
+    flush_front<ENGINE>(a, t, w, f, w.ring<ENGINE>(f), lane);
+    c -= ring_flush(ENGINE);
+  }
+  if (c) {
+    wave_sync();
+    if constexpr (ENGINE == 2 || ENGINE == 3)
+      compare_list_mfma<ENGINE>(a, t, t.fam[f], w.ring<ENGINE>(f), c, lane);
+    else if constexpr (ENGINE == 4)
+    {
+      __builtin_amdgcn_s_setprio(0);
+      seed_compare(a, t, t.fam[f], w.ring<ENGINE>(f), c, lane, w.vcnt + f * 64);
+      __builtin_amdgcn_s_setprio(1);
+    }
+    else
+    {
+      __builtin_amdgcn_s_setprio(0);
+      compare_list(a, t, t.fam[f], w.ring<ENGINE>(f), c, lane);
+      __builtin_amdgcn_s_setprio(1);
+    }
+    if (lane == 0) w.wstats[f] += c;
+  }
+  wave_sync();
+  if (lane == 0) w.counts[f] = 0;
+  wave_sync();
+}
+
 template <int NF, int ENGINE>
 __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w, uint32_t s,
-                                uint32_t cs, uint32_t ce, uint32_t lane) {
+                                uint32_t cs, uint32_t ce, uint32_t flag, uint32_t lane) {
   const uint2* __restrict__ planes = s ? a.planes1 : a.planes0;
   uint32_t nibs[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     nibs[f] = t.fam[f].nibs;
   }
-  if (lane < kMaxFam) w.counts[lane] = 0;
-  wave_sync();
   const uint32_t b_first = cs >> 5, b_last = (ce - 1) >> 5;
   const uint32_t cb = b_first << 5;  // ring entry offsets are from here
+  // This run's entries carry `flag`; the windows left in the buffers by the
+  // wave's previous run (the other flag) are marked as carried over.
+  if (lane == 0) {
+    w.runs[2 * flag] = cb;
+    w.runs[2 * flag + 1] = s;
+  }
+  if (lane < kMaxFam) w.counts[lane] = (w.counts[lane] & 0xFFFFu) * 0x10001u;
+  wave_sync();
   // Each lane loads one plane word per step, of block min(b, b_last + 1) (the
   // plane array extends past every scanned block), two steps ahead: the word
   // loaded in step k is stored to the wave's LDS stage in step k + 1 and read
@@ -970,36 +1025,17 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
           const uint32_t n = direct_family(a, t, t.fam[f], s, pw, bb, qa, qb, lane);
           if (lane == 0) w.wstats[f] += n;
         } else {
-          append_family<ENGINE>(a, t, w, f, s, pw, bb, qa, qb, cb, lane);
+          append_family<ENGINE>(a, t, w, f, pw, bb, qa, qb, cb, flag, lane);
         }
       }
     }
   }
+  // A buffer that still holds windows of the previous run is compared now
+  // (the next run reuses that run's flag); the others carry their windows
+  // into the wave's next run, where they fill whole batches.
 #pragma nounroll
-  for (uint32_t f = 0; f < NF; ++f) {
-    uint32_t c = uniform(w.counts[f]);
-    if (c > ring_flush(ENGINE)) {
-      flush_front<ENGINE>(a, t, w, f, s, w.ring<ENGINE>(f), cb, lane);
-      c -= ring_flush(ENGINE);
-    }
-    if (c) {
-      if constexpr (ENGINE == 2 || ENGINE == 3)
-        compare_list_mfma<ENGINE>(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane);
-      else if constexpr (ENGINE == 4)
-      {
-        __builtin_amdgcn_s_setprio(0);
-        seed_compare(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane, w.vcnt + f * 64);
-        __builtin_amdgcn_s_setprio(1);
-      }
-      else
-      {
-        __builtin_amdgcn_s_setprio(0);
-        compare_list(a, t, t.fam[f], s, w.ring<ENGINE>(f), c, cb, lane);
-        __builtin_amdgcn_s_setprio(1);
-      }
-      if (lane == 0) w.wstats[f] += c;
-    }
-  }
+  for (uint32_t f = 0; f < NF; ++f)
+    if (uniform(w.counts[f]) >> 16) drain_family<ENGINE>(a, t, w, f, lane);
   wave_sync();
 }
 
@@ -1040,7 +1076,8 @@ __device__ __forceinline__ void scan_kernel_body(
   w.wstats = reinterpret_cast<unsigned long long*>(my + kRings);
   w.pws = reinterpret_cast<uint32_t*>(my + kRings + kMaxFam * 8);
   w.counts = w.pws + NF * 64;
-  w.stage = reinterpret_cast<uint2*>(w.counts + kMaxFam);
+  w.runs = w.counts + kMaxFam;
+  w.stage = reinterpret_cast<uint2*>(w.runs + 4);
   // MFMA engines: kRecStage staged records and their count after the plane words
   bgo_hit* const rec_stage = ENGINE >= 2 ? reinterpret_cast<bgo_hit*>(w.stage + 64) : nullptr;
   uint32_t* const rec_cnt =
@@ -1055,14 +1092,17 @@ __device__ __forceinline__ void scan_kernel_body(
 #pragma unroll
     for (int f = 0; f < NF; ++f) w.vcnt[f * 64 + lane] = 0;
   }
-  const Tab t{fam,       g01,       gk,        gaux, gid,     gnm,  score_tab,
-              hits,      hit_count, btab,      rec_stage, rec_cnt, stab, sgix};
+  const Tab t{fam,       g01,       gk,        gaux,    gid,    gnm,  score_tab, hits,
+              hit_count, btab,      rec_stage, rec_cnt, w.runs, stab, sgix};
   // fp4 and VALU: priority 1 outside the batch compares, which run at 0 (the
   // scan, append and in-place passes are dependent chains; a compare has
   // many independent steps in flight).  VALU: 22 / 80 / 1024 mixed guides
   // 3.38 / 5.83 / 37.8 -> 3.27 / 5.43 / 35.8 ms.
   if constexpr (ENGINE == 3 || ENGINE == 1 || ENGINE == 4) __builtin_amdgcn_s_setprio(1);
-  if (lane < kMaxFam) w.wstats[lane] = 0;
+  if (lane < kMaxFam) {
+    w.wstats[lane] = 0;
+    w.counts[lane] = 0;
+  }
   if constexpr (ENGINE >= 2) {
     if (lane == 0) *rec_cnt = 0;
   }
@@ -1079,6 +1119,7 @@ __device__ __forceinline__ void scan_kernel_body(
   const uint32_t gw = blockIdx.x * n_waves + wave;
   const uint32_t k = __builtin_amdgcn_readfirstlane((gw + gw / kWorkCtrs) % kWorkCtrs);
   unsigned int* const my_counter = work_counter + k * kCtrStride;
+  uint32_t n_run = 0;  // runs of this wave so far; the parity flags their buffer entries
   for (;;) {
     uint32_t r = 0;
     if (lane == 0) r = atomicAdd(my_counter, 1u);
@@ -1097,8 +1138,13 @@ __device__ __forceinline__ void scan_kernel_body(
     const uint32_t s = slot ? a.slot_strand1 : a.slot_strand0;
     const uint32_t cs = a.valid_begin + run * a.chunk;
     const uint32_t ce = min(cs + a.chunk, a.valid_end);
-    scan_run<NF, ENGINE>(a, t, w, s, cs, ce, lane);
+    scan_run<NF, ENGINE>(a, t, w, s, cs, ce, n_run & 1u, lane);
+    ++n_run;
   }
+  // the windows the last runs left in the buffers
+#pragma nounroll
+  for (uint32_t f = 0; f < NF; ++f)
+    if (uniform(w.counts[f]) & 0xFFFFu) drain_family<ENGINE>(a, t, w, f, lane);
   if constexpr (ENGINE >= 2) flush_records(a, t, lane);
   wave_sync();
   if (lane < NF && w.wstats[lane]) atomicAdd(&fam_windows[lane], w.wstats[lane]);
@@ -1399,9 +1445,9 @@ int resolve_tuning(bgo_plan* P, const bgo_tuning* t) {
   if (t) r = *t;
   if (r.chunk_positions <= 0) r.chunk_positions = 131072;
   r.chunk_positions = std::max(64, (r.chunk_positions + 63) / 64 * 64);
-  // ring entries hold 24-bit offsets from the 32-aligned chunk start
-  if (r.chunk_positions > (1 << 24) - 64)
-    return fail(BGO_ERR_ARG, "chunk_positions must be <= 2^24 - 64");
+  // ring entries hold 23-bit offsets from the 32-aligned chunk start
+  if (r.chunk_positions > (1 << 23) - 64)
+    return fail(BGO_ERR_ARG, "chunk_positions must be <= 2^23 - 64");
   if (r.waves_per_block <= 0) r.waves_per_block = 4;
   if (r.waves_per_block != 1 && r.waves_per_block != 2 && r.waves_per_block != 4)
     return fail(BGO_ERR_ARG, "waves_per_block must be 1, 2 or 4");
