@@ -23,7 +23,7 @@ from .batched import DeviceGenome, GuideSpec, Plan
 
 DEFAULT_SPACE = {
     "engine": [1, 2, 3, 4],
-    "chunk_positions": [32768, 131072],
+    "chunk_positions": [16384, 32768, 131072],
     "waves_per_block": [2, 4],
     "blocks_per_cu": [0],
     "interleave": [0, 1],
