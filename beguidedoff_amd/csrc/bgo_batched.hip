@@ -667,11 +667,13 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
           asm volatile("" : "+v"(any));
         }
         if (ballot((any & kField3Mask) != 0)) {
-          // the rare triple is redone from reloaded operands (the registers
-          // already hold the next triple)
-          fp4_tile(a, t, L, n, lane, kAT, A, btile(ti), F.g_begin + 32 * ti);
-          fp4_tile(a, t, L, n, lane, kAT, A, btile(ti + 1), F.g_begin + 32 * ti + 32);
-          fp4_tile(a, t, L, n, lane, kAT, A, btile(ti + 2), F.g_begin + 32 * ti + 64);
+          // Rare: the guide tiles whose field has a set bit somewhere in the
+          // wave are redone from reloaded operands (the registers already
+          // hold the next triple).
+#pragma nounroll
+          for (uint32_t q = 0; q < 3; ++q)
+            if (ballot((any >> (6 + 7 * q)) & 1u))
+              fp4_tile(a, t, L, n, lane, kAT, A, btile(ti + q), F.g_begin + 32 * (ti + q));
         }
       }
       // the last one or two guide tiles (b0, b1 hold tiles ti, ti + 1)
