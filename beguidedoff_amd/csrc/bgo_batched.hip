@@ -65,7 +65,7 @@ struct Fam {
   uint32_t direct;          // 1: windows compared in place, without the buffer
   uint32_t seed_start;      // seed engine: word offset of the key starts in the block's table
   uint32_t seed_ent;        // seed engine: word offset of the entries in the block's table
-  uint32_t pad;
+  uint32_t seed_n;          // seed engine: the family's table entries (bucket pads included)
 };
 
 struct Args {
@@ -729,12 +729,16 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
 // Seed engine (ENGINE 4): a window within 4 mismatches of a guide matches it
 // exactly in at least one of the five 4-base blocks of the 20 compared bases
 // (pigeonhole), so each window looks up, per block, the family's guides with
-// the same 4 bases there and verifies only those.  Per family and block j the
-// block's seed table (shared by the waves of a block, in LDS) holds 257 key
-// starts and then, sorted by key (the 8 plane bits of the block's 4 bases),
-// one word per guide: its other 16 bases, b0 plane bits low and b1 plane bits
-// high.  A table word's guide is in sgix at the same word offset.  Only
-// families whose budgets are all <= 4 use it.
+// the same 4 bases there and verifies only those.  Per family, the seed table
+// (shared by the waves of a block, in LDS) holds for each block j 257 key
+// starts (the key is the 8 plane bits of the block's 4 bases), then the
+// entries of all blocks: per block and key, the guides with that key, padded
+// to an even count, two guides per pair of words: the first word holds both
+// guides' b0 plane bits of their other 16 bases (first guide low), the second
+// their b1 plane bits.  A start is the entry index of its key's first guide,
+// with bit 0 set when the key's guide count is odd (the last entry is then a
+// pad).  Entry e's guide is in sgix at the same offset.  Only families whose
+// budgets are all <= 4 use it.
 constexpr uint32_t kSeedBlocks = 5;
 constexpr uint32_t kSeedKeys = 256;
 constexpr uint32_t kSeedPad = 4;  // spare words after a family's entries
@@ -777,32 +781,35 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, c
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
   const uint32_t pos = entry_pos(t.runs, e), s = entry_strand(t.runs, e);
   const uint32_t kmax = F.kmax;
-  const uint32_t n_g = F.g_live - F.g_begin;
   const uint32_t* st = t.stab + F.seed_start;
   const uint32_t* en = t.stab + F.seed_ent;
+  const uint32_t last = F.seed_n;  // reads of entries last..last + 3 hit the kSeedPad spare words
 #pragma unroll
   for (uint32_t j = 0; j < kSeedBlocks; ++j) {
     const uint32_t key = ((W0 >> (4 * j)) & 0xFu) | (((W1 >> (4 * j)) & 0xFu) << 4);
-    const uint32_t wo = seed_other(W0, j) | (seed_other(W1, j) << 16);
+    // the window's other 16 bases, per plane, in both halves of a word
+    const uint32_t wl = seed_other(W0, j) * 0x10001u, wh = seed_other(W1, j) * 0x10001u;
     const uint32_t* sj = st + j * (kSeedKeys + 1);
-    const uint32_t beg = sj[key];
-    const uint32_t end = has ? sj[key + 1] : beg;
+    const uint32_t s0 = sj[key], s1 = sj[key + 1];
+    const uint32_t beg = s0 & ~1u;
+    const uint32_t end = has ? (s1 & ~1u) - (s0 & 1u) : beg;
     verified += end - beg;
-    // Four entries per step, read together.  A lane whose key's entries are
-    // done goes on reading the following ones while the wave's longest walk
-    // lasts; only the rare candidates are checked to lie in the lane's own
-    // key (i + q < end), and the reads stay in the family's table (clamped to
-    // its last entries, which end with kSeedPad spare words; a clamped read
-    // is never in the lane's key).
-    const uint32_t last = kSeedBlocks * n_g - 1;
+    // Four entries (two word pairs) per step, read together.  A lane whose
+    // key's entries are done goes on reading the following ones while the
+    // wave's longest walk lasts; only the rare candidates are checked to lie
+    // in the lane's own key (i + q < end), and the reads stay in the
+    // family's table (clamped to its end, which is followed by kSeedPad
+    // spare words; a clamped read is never in the lane's key).
     for (uint32_t i = beg; ballot(i < end); i += 4) {
-      const uint32_t b = min(j * n_g + i, last);
-      uint32_t p[4];
-#pragma unroll
-      for (uint32_t q = 0; q < 4; ++q) {
-        const uint32_t d = en[b + q] ^ wo;
-        p[q] = static_cast<uint32_t>(__popc((d | (d >> 16)) & 0xFFFFu));
-      }
+      const uint32_t b = min(i, last);
+      const uint2 e01 = *reinterpret_cast<const uint2*>(en + b);  // entries b, b + 1 (b even)
+      const uint2 e23 = *reinterpret_cast<const uint2*>(en + b + 2);
+      const uint32_t m01 = (e01.x ^ wl) | (e01.y ^ wh);
+      const uint32_t m23 = (e23.x ^ wl) | (e23.y ^ wh);
+      const uint32_t p[4] = {static_cast<uint32_t>(__popc(m01 & 0xFFFFu)),
+                             static_cast<uint32_t>(__popc(m01 >> 16)),
+                             static_cast<uint32_t>(__popc(m23 & 0xFFFFu)),
+                             static_cast<uint32_t>(__popc(m23 >> 16))};
       if (ballot(min(min(p[0], p[1]), min(p[2], p[3])) <= kmax)) {
 #pragma nounroll
         for (uint32_t q = 0; q < 4; ++q) {
@@ -1536,14 +1543,18 @@ int validate_guides(const bgo_guide_desc* guides, int32_t n_guides) {
 // tables, grow the device allocation if needed and upload on the plan stream.
 // The previous launch must be complete (callers synchronise first).
 // Seed engine tables.  A family part holds at most kSeedMaxGuides guides,
-// so that its table (5 x 257 key starts, 5 words per guide and 4 spare
-// words, 87,076 bytes at 4,096 guides) and one family's wave buffers for a 1,024-thread block fit
-// the 160 KiB of LDS of a CU.
+// so that its table (5 x 257 key starts, 5 entries per guide, at most 5 x 256
+// pad entries, 4 spare words and alignment: at most 92,220 bytes at 4,096 guides) and one
+// family's wave buffers for a 1,024-thread block fit the 160 KiB of LDS of a
+// CU.
 constexpr size_t kSeedMaxGuides = 4096;
 constexpr size_t kSeedLdsBytes = 160 * 1024;
 
+// Largest table of a family part of n_guides guides (every key's guide count
+// padded to even).
 size_t seed_image_bytes(uint32_t n_guides) {
-  return (bk::kSeedBlocks * (bk::kSeedKeys + 1) + bk::kSeedBlocks * size_t{n_guides} + bk::kSeedPad) * 4;
+  return (bk::kSeedBlocks * (bk::kSeedKeys + 1) + 3 +
+          bk::kSeedBlocks * (size_t{n_guides} + bk::kSeedKeys) + bk::kSeedPad + 3) * 4;
 }
 
 // LDS of a seed-engine block: the table (its start rounded to 256 bytes for
@@ -1574,12 +1585,12 @@ std::vector<int> seed_groups(const std::vector<uint32_t>& part_guides) {
   return first;
 }
 
-// Per launch group: the image copied to LDS (per family, then per block j of
-// the 20 compared bases: 257 key starts, then one word per guide sorted by
-// the 8-bit key of its bases 4j..4j+3: the other 16 bases, b0 plane bits low
-// and b1 high), padded to 16 bytes, followed by as many words naming the
-// guide behind each entry word.  Fam::seed_start / seed_ent are word offsets
-// into the group's image.
+// Per launch group: the image copied to LDS (per family the layout described
+// at kSeedBlocks: 5 x 257 key starts, then the entries of blocks 0..4, keyed
+// by the 8-bit key of bases 4j..4j+3, as word pairs of two guides' other 16
+// bases per plane), padded to 16 bytes, followed by as many words naming the
+// guide behind each entry.  Fam::seed_start / seed_ent are word offsets into
+// the group's image; Fam::seed_n counts the family's entries.
 int build_seed_tables(bgo_plan* P) {
   P->h_btab.clear();
   for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
@@ -1589,29 +1600,45 @@ int build_seed_tables(bgo_plan* P) {
     std::vector<uint32_t> img, gix;
     for (int f = f0; f < f_end; ++f) {
       Fam& F = P->fams[f];
-      const uint32_t n = F.g_live - F.g_begin;
       F.seed_start = static_cast<uint32_t>(img.size());
-      F.seed_ent = F.seed_start + bk::kSeedBlocks * (bk::kSeedKeys + 1);
-      img.resize(F.seed_ent + bk::kSeedBlocks * n + bk::kSeedPad, 0u);
-      gix.resize(img.size(), 0u);
+      // entries start 16-byte aligned (word pairs are read as 8-byte words)
+      F.seed_ent = (F.seed_start + bk::kSeedBlocks * (bk::kSeedKeys + 1) + 3) / 4 * 4;
+      img.resize(F.seed_ent, 0u);
+      uint32_t cur = 0;  // entries so far (always even)
       for (uint32_t j = 0; j < bk::kSeedBlocks; ++j) {
-        uint32_t* st = img.data() + F.seed_start + j * (bk::kSeedKeys + 1);
-        uint32_t cnt[bk::kSeedKeys] = {};
-        auto key_of = [&](uint32_t g) {
+        std::vector<std::vector<uint32_t>> bucket(bk::kSeedKeys);
+        for (uint32_t g = F.g_begin; g < F.g_live; ++g) {  // guide order within a key
           const uint32_t p0 = P->h_g01[2 * g], p1 = P->h_g01[2 * g + 1];
-          return ((p0 >> (4 * j)) & 0xFu) | (((p1 >> (4 * j)) & 0xFu) << 4);
-        };
-        for (uint32_t g = F.g_begin; g < F.g_live; ++g) ++cnt[key_of(g)];
-        st[0] = 0;
-        for (uint32_t k = 0; k < bk::kSeedKeys; ++k) st[k + 1] = st[k] + cnt[k];
-        uint32_t fill[bk::kSeedKeys];
-        for (uint32_t k = 0; k < bk::kSeedKeys; ++k) fill[k] = st[k];
-        for (uint32_t g = F.g_begin; g < F.g_live; ++g) {  // stable: guide order within a key
-          const uint32_t w = F.seed_ent + j * n + fill[key_of(g)]++;
-          img[w] = bk::seed_other(P->h_g01[2 * g], j) | (bk::seed_other(P->h_g01[2 * g + 1], j) << 16);
-          gix[w] = g;
+          bucket[((p0 >> (4 * j)) & 0xFu) | (((p1 >> (4 * j)) & 0xFu) << 4)].push_back(g);
+        }
+        for (uint32_t k = 0; k <= bk::kSeedKeys; ++k) {
+          const uint32_t odd = k < bk::kSeedKeys ? static_cast<uint32_t>(bucket[k].size() & 1u) : 0u;
+          img[F.seed_start + j * (bk::kSeedKeys + 1) + k] = cur | odd;
+          if (k == bk::kSeedKeys) break;
+          auto& b = bucket[k];
+          // The pad entry repeats the key's last guide, so that it passes the
+          // mismatch test exactly when that guide does, in the same step (it is
+          // never in its key's range).
+          if (odd) b.push_back(b.back());
+          for (size_t e = 0; e < b.size(); e += 2) {
+            uint32_t lo = 0, hi = 0;
+            for (uint32_t h = 0; h < 2; ++h) {
+              const uint32_t g = b[e + h];
+              lo |= bk::seed_other(P->h_g01[2 * g], j) << (16 * h);
+              hi |= bk::seed_other(P->h_g01[2 * g + 1], j) << (16 * h);
+            }
+            img.push_back(lo);
+            img.push_back(hi);
+            gix.resize(img.size(), 0u);
+            gix[F.seed_ent + cur + e] = b[e];
+            gix[F.seed_ent + cur + e + 1] = b[e + 1];
+          }
+          cur += static_cast<uint32_t>(b.size());
         }
       }
+      F.seed_n = cur;
+      img.resize((img.size() + bk::kSeedPad + 3) / 4 * 4, 0u);
+      gix.resize(img.size(), 0u);
     }
     img.resize((img.size() + 3) / 4 * 4, 0u);
     gix.resize(img.size(), 0u);

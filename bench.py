@@ -68,11 +68,13 @@ def seed_ops(L_pos: int, n_strands: int, families, verified) -> int:
     """32-bit lane-ops of the seed-and-verify formulation for one launch: the
     window and PAM extraction and PAM tests of the bit-plane scan, per
     (window, 4-base block) the block's key and the window's other 16 bases
-    (10 ops), and per verified guide-table entry xor, fold, mask, popcount and
-    compare (5 ops); `verified` counts the entries on the device."""
+    in both halves of a word per plane (10 ops), and per pair of verified
+    guide-table entries (the table holds two guides per word) one xor per
+    plane and their or, then per entry its half, popcount and compare (9 ops
+    per pair); `verified` counts the entries on the device."""
     ops = n_strands * L_pos * 2 + sum(n_strands * L_pos * 4 for _ in families)
     ops += sum(5 * 10 * w for _, w in families)
-    ops += 5 * int(sum(verified))
+    ops += 9 * int(sum(verified)) // 2
     return ops
 
 
@@ -100,7 +102,7 @@ UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.702, "mfma": 0.506}}
 
 # The seed engine on 8192 NGG guides (two 4,096-guide tables, two launches
 # per step): VALU busy from SQ_INSTS_VALU over GRBM_GUI_ACTIVE SIMD-cycles.
-SEED_BUSY = {("hg38_8k", 4, 4): ({"valu": 0.894},
+SEED_BUSY = {("hg38_8k", 4, 4): ({"valu": 0.908},
                                  "profiles/seed_scan_kernel_hg38_8k_4mm.md")}
 
 
