@@ -741,7 +741,8 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
 // budgets are all <= 4 use it.
 constexpr uint32_t kSeedBlocks = 5;
 constexpr uint32_t kSeedKeys = 256;
-constexpr uint32_t kSeedPad = 4;  // spare words after a family's entries
+constexpr uint32_t kSeedStep = 8;  // table entries (word pairs x 2) read per step of a walk
+constexpr uint32_t kSeedPad = kSeedStep;  // spare words after a family's entries
 
 // The 16 bits of a 20-bit plane word outside bases 4j..4j+3, packed in order.
 __host__ __device__ inline uint32_t seed_other(uint32_t w, uint32_t j) {
@@ -783,36 +784,53 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, c
   const uint32_t kmax = F.kmax;
   const uint32_t* st = t.stab + F.seed_start;
   const uint32_t* en = t.stab + F.seed_ent;
-  const uint32_t last = F.seed_n;  // reads of entries last..last + 3 hit the kSeedPad spare words
+  const uint32_t last = F.seed_n;  // reads of entries last.. hit the kSeedPad spare words
+  // every block's key starts, read together up front
+  uint32_t s0s[kSeedBlocks], s1s[kSeedBlocks];
 #pragma unroll
   for (uint32_t j = 0; j < kSeedBlocks; ++j) {
     const uint32_t key = ((W0 >> (4 * j)) & 0xFu) | (((W1 >> (4 * j)) & 0xFu) << 4);
+    const uint32_t* sj = st + j * (kSeedKeys + 1);
+    s0s[j] = sj[key];
+    s1s[j] = sj[key + 1];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kSeedBlocks; ++j) {
     // the window's other 16 bases, per plane, in both halves of a word
     const uint32_t wl = seed_other(W0, j) * 0x10001u, wh = seed_other(W1, j) * 0x10001u;
-    const uint32_t* sj = st + j * (kSeedKeys + 1);
-    const uint32_t s0 = sj[key], s1 = sj[key + 1];
+    const uint32_t s0 = s0s[j], s1 = s1s[j];
     const uint32_t beg = s0 & ~1u;
     const uint32_t end = has ? (s1 & ~1u) - (s0 & 1u) : beg;
     verified += end - beg;
-    // Four entries (two word pairs) per step, read together.  A lane whose
+    // kSeedStep entries (word pairs of two) per step, read together.  A lane whose
     // key's entries are done goes on reading the following ones while the
     // wave's longest walk lasts; only the rare candidates are checked to lie
     // in the lane's own key (i + q < end), and the reads stay in the
     // family's table (clamped to its end, which is followed by kSeedPad
     // spare words; a clamped read is never in the lane's key).
-    for (uint32_t i = beg; ballot(i < end); i += 4) {
-      const uint32_t b = min(i, last);
-      const uint2 e01 = *reinterpret_cast<const uint2*>(en + b);  // entries b, b + 1 (b even)
-      const uint2 e23 = *reinterpret_cast<const uint2*>(en + b + 2);
-      const uint32_t m01 = (e01.x ^ wl) | (e01.y ^ wh);
-      const uint32_t m23 = (e23.x ^ wl) | (e23.y ^ wh);
-      const uint32_t p[4] = {static_cast<uint32_t>(__popc(m01 & 0xFFFFu)),
-                             static_cast<uint32_t>(__popc(m01 >> 16)),
-                             static_cast<uint32_t>(__popc(m23 & 0xFFFFu)),
-                             static_cast<uint32_t>(__popc(m23 >> 16))};
-      if (ballot(min(min(p[0], p[1]), min(p[2], p[3])) <= kmax)) {
+    const uint2* pend = reinterpret_cast<const uint2*>(en + end);
+    const uint2* plast = reinterpret_cast<const uint2*>(en + last);
+    for (const uint2* pi = reinterpret_cast<const uint2*>(en + beg); ballot(pi < pend);
+         pi += kSeedStep / 2) {
+      const uint2* pb = pi < plast ? pi : plast;  // entries 2 (pb - en) .. (even)
+      // per word pair, each half's count: the high half's, and the whole
+      // word's less it
+      uint32_t p[kSeedStep];
+      uint32_t lo = 32;
+#pragma unroll
+      for (uint32_t q = 0; q < kSeedStep / 2; ++q) {
+        const uint2 e = pb[q];
+        const uint32_t m = (e.x ^ wl) | (e.y ^ wh);
+        const uint32_t h = static_cast<uint32_t>(__popc(m >> 16));
+        p[2 * q] = static_cast<uint32_t>(__popc(m)) - h;
+        p[2 * q + 1] = h;
+        lo = min(lo, min(p[2 * q], h));
+      }
+      if (ballot(lo <= kmax)) {
+        const uint32_t b = static_cast<uint32_t>(reinterpret_cast<const uint32_t*>(pb) - en);
+        const uint32_t i = static_cast<uint32_t>(reinterpret_cast<const uint32_t*>(pi) - en);
 #pragma nounroll
-        for (uint32_t q = 0; q < 4; ++q) {
+        for (uint32_t q = 0; q < kSeedStep; ++q) {
           const bool cand = p[q] <= kmax && i + q < end;
           if (ballot(cand)) seed_exact(a, t, s, lane, W0, W1, pos, j, F.seed_ent + b + q, cand);
         }
@@ -1544,7 +1562,7 @@ int validate_guides(const bgo_guide_desc* guides, int32_t n_guides) {
 // The previous launch must be complete (callers synchronise first).
 // Seed engine tables.  A family part holds at most kSeedMaxGuides guides,
 // so that its table (5 x 257 key starts, 5 entries per guide, at most 5 x 256
-// pad entries, 4 spare words and alignment: at most 92,220 bytes at 4,096 guides) and one
+// pad entries, 8 spare words and alignment: at most 92,236 bytes at 4,096 guides) and one
 // family's wave buffers for a 1,024-thread block fit the 160 KiB of LDS of a
 // CU.
 constexpr size_t kSeedMaxGuides = 4096;
@@ -1555,6 +1573,60 @@ constexpr size_t kSeedLdsBytes = 160 * 1024;
 size_t seed_image_bytes(uint32_t n_guides) {
   return (bk::kSeedBlocks * (bk::kSeedKeys + 1) + 3 +
           bk::kSeedBlocks * (size_t{n_guides} + bk::kSeedKeys) + bk::kSeedPad + 3) * 4;
+}
+
+// Guide counts of the parts a family of g guides is split into for the seed
+// engine: as few parts as hold kSeedMaxGuides each, of equal sizes (the
+// first g % n parts one guide larger).
+std::vector<uint32_t> seed_part_sizes(size_t g) {
+  const size_t n = std::max<size_t>(1, (g + kSeedMaxGuides - 1) / kSeedMaxGuides);
+  std::vector<uint32_t> sizes(n, static_cast<uint32_t>(g / n));
+  for (size_t p = 0; p < g % n; ++p) ++sizes[p];
+  return sizes;
+}
+
+// A family's guides (indices into guides, in order) split into seed-engine
+// parts of seed_part_sizes.  A window's walk in a part lasts as long as the
+// longest key list among the wave's windows, so each guide goes to the part
+// (with room left) whose lists for the guide's five block keys are shortest
+// so far (on a tie, the part with fewer guides), which evens out the parts'
+// list lengths.
+std::vector<std::vector<int32_t>> seed_split(const bgo_guide_desc* guides,
+                                             const std::vector<int32_t>& members) {
+  const std::vector<uint32_t> sizes = seed_part_sizes(members.size());
+  const size_t n = sizes.size();
+  std::vector<std::vector<int32_t>> parts(n);
+  if (n == 1) {
+    parts[0] = members;
+    return parts;
+  }
+  std::vector<uint32_t> len(n * bk::kSeedBlocks * bk::kSeedKeys, 0u);
+  for (int32_t i : members) {
+    uint32_t key[bk::kSeedBlocks];
+    for (uint32_t j = 0; j < bk::kSeedBlocks; ++j) {
+      uint32_t k = 0;
+      for (uint32_t b = 0; b < 4; ++b) {  // the key's plane bits, as the table builds it
+        const uint32_t code = static_cast<uint32_t>(guides[i].seq2bit >> (2 * (4 * j + b))) & 3u;
+        k |= ((code & 1u) << b) | ((code >> 1) << (4 + b));
+      }
+      key[j] = k;
+    }
+    size_t best = n;
+    uint32_t best_load = 0;
+    for (size_t p = 0; p < n; ++p) {
+      if (parts[p].size() >= sizes[p]) continue;
+      uint32_t load = 0;
+      for (uint32_t j = 0; j < bk::kSeedBlocks; ++j)
+        load += len[(p * bk::kSeedBlocks + j) * bk::kSeedKeys + key[j]];
+      if (best == n || load < best_load ||
+          (load == best_load && parts[p].size() < parts[best].size()))
+        best = p, best_load = load;
+    }
+    parts[best].push_back(i);
+    for (uint32_t j = 0; j < bk::kSeedBlocks; ++j)
+      ++len[(best * bk::kSeedBlocks + j) * bk::kSeedKeys + key[j]];
+  }
+  return parts;
 }
 
 // LDS of a seed-engine block: the table (its start rounded to 256 bytes for
@@ -1689,8 +1761,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
       fp4 += pass * (8.1 + 4.0 * ((g + 31) / 32));
       seed += pass * (36.0 + 27.0 * g / 1024.0);
       for (int32_t i : members[f]) seed_ok = seed_ok && guides[i].max_mm <= 4;
-      for (size_t b = 0; b < static_cast<size_t>(g); b += kSeedMaxGuides)
-        parts.push_back(static_cast<uint32_t>(std::min<size_t>(g - b, kSeedMaxGuides)));
+      for (uint32_t n : seed_part_sizes(static_cast<size_t>(g))) parts.push_back(n);
     }
     seed += 0.9 * (static_cast<double>(seed_groups(parts).size()) - 1.0);
     P->tune.engine = fp4 < valu ? 3 : 1;
@@ -1708,9 +1779,8 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     std::vector<std::vector<int32_t>> parts;
     std::vector<Fam> part_proto;
     for (int f = 0; f < n_fam; ++f)
-      for (size_t b = 0; b < members[f].size(); b += kSeedMaxGuides) {
-        const size_t e = std::min(members[f].size(), b + kSeedMaxGuides);
-        parts.emplace_back(members[f].begin() + b, members[f].begin() + e);
+      for (auto& part : seed_split(guides, members[f])) {
+        parts.push_back(std::move(part));
         part_proto.push_back(proto[f]);
       }
     members.swap(parts);

@@ -102,7 +102,7 @@ UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.702, "mfma": 0.506}}
 
 # The seed engine on 8192 NGG guides (two 4,096-guide tables, two launches
 # per step): VALU busy from SQ_INSTS_VALU over GRBM_GUI_ACTIVE SIMD-cycles.
-SEED_BUSY = {("hg38_8k", 4, 4): ({"valu": 0.908},
+SEED_BUSY = {("hg38_8k", 4, 4): ({"valu": 0.849},
                                  "profiles/seed_scan_kernel_hg38_8k_4mm.md")}
 
 
@@ -142,7 +142,7 @@ def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, f
     seed_busy = SEED_BUSY.get((workload, engine, max_mm))
     if engine == 4 and seed_busy:
         out["binding_unit"] = (
-            f"VALU {seed_busy[0]['valu']:.0%} busy; the counted lane-ops are 56% of the issued "
+            f"VALU {seed_busy[0]['valu']:.0%} busy; the counted lane-ops are 68% of the issued "
             "VALU lane-slots: a wave walks its 64 windows' guide lists for as long as the "
             "longest of them")
         out["unit_busy"] = dict(seed_busy[0], source=seed_busy[1])
