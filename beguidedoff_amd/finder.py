@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import ctypes as C
 import logging
+import threading
 from collections import defaultdict
 from dataclasses import dataclass
 from pathlib import Path
@@ -205,6 +206,10 @@ class beguidedoff:
         self._chrom_names_arr = np.array(self.chrom_names, dtype=object)
         self._prepare_n_masks()
         self._genome_handle = None   # batched engine: device genome, built lazily
+        # the device-table path's plan, reused by the next search (its buffers
+        # and stream stay allocated); the lock keeps one search on it at a time
+        self._table_plan = None
+        self._table_lock = threading.Lock()
 
     @classmethod
     def host_view(cls, genome: dict, max_mismatches: int = 4, n_top_offtargets: int = 3,
@@ -230,6 +235,8 @@ class beguidedoff:
         self._chrom_names_arr = np.array(self.chrom_names, dtype=object)
         self._prepare_n_masks()
         self._genome_handle = None
+        self._table_plan = None
+        self._table_lock = threading.Lock()
         return self
 
     def assemble(self, guides: List[GuideInput], raw_by_key: Dict[tuple, RawHits],
@@ -469,7 +476,7 @@ class beguidedoff:
         N on the device; only the rows come back.  None when the launch's
         sites overflow its buffer or a key exceeds the escalation ceiling on a
         strand: the host path then runs (scan_capped + assemble_results)."""
-        from .batched import DEFAULT_MAX_RECORDS, GuideSpec, Plan, default_capacity
+        from .batched import DEFAULT_MAX_RECORDS, GuideSpec, default_capacity
 
         n_keys = len(scan_guides)
         if n_keys == 0:
@@ -477,40 +484,62 @@ class beguidedoff:
         specs = [GuideSpec(sg.guide_seq, sg.pam, self.max_mismatches) for sg in scan_guides]
         limit = self.max_records if self.max_records is not None else DEFAULT_MAX_RECORDS
         cap = min(limit, default_capacity(n_keys))
-        plan = Plan(self._device_genome(), specs, tuning=self.tuning, hit_capacity=cap)
-        try:
-            plan.launch()
-            n, _ms = plan.wait()
-            if n > cap:
+        with self._table_lock:
+            try:
+                return self._device_table_on_plan(guides, scan_guides, mapping, specs, cap,
+                                                  count_score_threshold, return_all)
+            except BaseException:
+                self._close_table_plan()  # its state after a failure is not reused
+                raise
+
+    def _close_table_plan(self) -> None:
+        if self._table_plan is not None:
+            self._table_plan.close()
+            self._table_plan = None
+
+    def _device_table_on_plan(self, guides, scan_guides, mapping, specs, cap,
+                              count_score_threshold, return_all) -> Optional[pd.DataFrame]:
+        from .batched import Plan
+
+        n_keys = len(scan_guides)
+        plan = self._table_plan
+        if plan is None:
+            plan = self._table_plan = Plan(self._device_genome(), specs, tuning=self.tuning,
+                                           hit_capacity=cap)
+        else:
+            plan.set_guides(specs)
+            if plan.hit_capacity != cap:
+                plan.set_hit_capacity(cap)
+        plan.launch()
+        # the host tables of the assembly are built while the scan runs
+        keys, inputs, gid, n_ids, input_index = self._assembly_tables(guides, scan_guides, mapping)
+        ceiling = (self.max_results_per_guide if self.max_results_per_guide is not None
+                   else _MAX_RESULTS_PER_GUIDE)
+        names = self.chrom_names
+        rank_of = {nm: i for i, nm in enumerate(sorted(set(names)))}
+        name_rank = np.array([rank_of[nm] for nm in names], dtype=np.uint32)
+        top = max(int(self.n_top_offtargets), 0)  # <= 0: no rows, as the host path
+        n, _ms = plan.wait()
+        if n > cap:
+            return None
+        rows_cap = max(1, n_ids * top) if not return_all else max(1, n * len(inputs))
+        below = np.zeros(max(n_ids, 1), dtype=np.int64)
+        while True:
+            rows = np.zeros(rows_cap, dtype=_lib.ASM_ROW_DTYPE)
+            n_rows, mx = C.c_int64(0), C.c_int64(0)
+            _lib.check(_lib.load().bgo_plan_assemble(
+                plan.handle, _lib.ptr(keys), n_keys, _lib.ptr(inputs), len(inputs), n_ids,
+                _lib.ptr(self.chrom_offsets), _lib.ptr(self.chrom_lengths),
+                _lib.ptr(name_rank), len(names), top, int(bool(return_all)),
+                float(count_score_threshold) if count_score_threshold is not None else 0.0,
+                int(count_score_threshold is not None), int(ceiling), _lib.ptr(rows), rows_cap,
+                C.byref(n_rows), _lib.ptr(below), C.byref(mx)))
+            if mx.value > ceiling:
                 return None
-            keys, inputs, gid, n_ids, input_index = self._assembly_tables(guides, scan_guides,
-                                                                          mapping)
-            ceiling = (self.max_results_per_guide if self.max_results_per_guide is not None
-                       else _MAX_RESULTS_PER_GUIDE)
-            names = self.chrom_names
-            rank_of = {nm: i for i, nm in enumerate(sorted(set(names)))}
-            name_rank = np.array([rank_of[nm] for nm in names], dtype=np.uint32)
-            top = max(int(self.n_top_offtargets), 0)  # <= 0: no rows, as the host path
-            rows_cap = max(1, n_ids * top) if not return_all else max(1, n * len(inputs))
-            below = np.zeros(max(n_ids, 1), dtype=np.int64)
-            while True:
-                rows = np.zeros(rows_cap, dtype=_lib.ASM_ROW_DTYPE)
-                n_rows, mx = C.c_int64(0), C.c_int64(0)
-                _lib.check(_lib.load().bgo_plan_assemble(
-                    plan.handle, _lib.ptr(keys), n_keys, _lib.ptr(inputs), len(inputs), n_ids,
-                    _lib.ptr(self.chrom_offsets), _lib.ptr(self.chrom_lengths),
-                    _lib.ptr(name_rank), len(names), top, int(bool(return_all)),
-                    float(count_score_threshold) if count_score_threshold is not None else 0.0,
-                    int(count_score_threshold is not None), int(ceiling), _lib.ptr(rows), rows_cap,
-                    C.byref(n_rows), _lib.ptr(below), C.byref(mx)))
-                if mx.value > ceiling:
-                    return None
-                if n_rows.value <= rows_cap:
-                    rows = rows[: n_rows.value]
-                    break
-                rows_cap = int(n_rows.value)
-        finally:
-            plan.close()
+            if n_rows.value <= rows_cap:
+                rows = rows[: n_rows.value]
+                break
+            rows_cap = int(n_rows.value)
         return self._rows_table(guides, scan_guides, rows, gid, below[:n_ids], input_index)
 
     def _assembly_tables(self, guides, scan_guides, mapping):
@@ -530,22 +559,30 @@ class beguidedoff:
                             for sg in scan_guides]
         keys["ot_strand"] = [0 if sg.on_target_strand == "+" else
                              (1 if sg.on_target_strand == "-" else -9) for sg in scan_guides]
-        pairs = [(k, input_index[id(g)]) for k, sg in enumerate(scan_guides) for g in mapping[sg]]
-        counts = np.bincount([k for k, _ in pairs], minlength=len(scan_guides))
+        counts = np.array([len(mapping[sg]) for sg in scan_guides], dtype=np.int64)
         keys["n_inputs"] = counts
         keys["first_input"] = np.cumsum(counts) - counts
-        inputs = np.zeros(len(pairs), dtype=_lib.ASM_INPUT_DTYPE)
+        inp_of_pair = np.fromiter((input_index[id(g)] for sg in scan_guides for g in mapping[sg]),
+                                  dtype=np.int64, count=int(counts.sum()))
+        # per input guide: its window mask and editable base (0 = A, 1 = C)
         win_of: Dict[tuple, int] = {}
-        for j, (_k, i) in enumerate(pairs):
-            g = guides[i]
+        mask_in = np.zeros(len(guides), dtype=np.uint32)
+        base_in = np.zeros(len(guides), dtype=np.uint8)
+        for i, g in enumerate(guides):
             base = EDITABLE_BASE.get(g.editor_type)
             if base is None:
                 raise ValueError(f"Unknown editor_type: {g.editor_type}")
             w = (g.edit_window_start, g.edit_window_end)
-            if w not in win_of:
-                win_of[w] = edit_window_mask(*w)
-            inputs[j] = (gid[i], win_of[w], {"A": 0, "C": 1}[base], j)
-        inp_of_pair = np.array([i for _, i in pairs], dtype=np.int64)
+            m = win_of.get(w)
+            if m is None:
+                m = win_of[w] = edit_window_mask(*w)
+            mask_in[i] = m
+            base_in[i] = 0 if base == "A" else 1
+        inputs = np.zeros(inp_of_pair.size, dtype=_lib.ASM_INPUT_DTYPE)
+        inputs["id"] = gid[inp_of_pair]
+        inputs["window"] = mask_in[inp_of_pair]
+        inputs["target"] = base_in[inp_of_pair]
+        inputs["order"] = np.arange(inp_of_pair.size)
         return keys, inputs, gid, len(id_of), inp_of_pair
 
     def _rows_table(self, guides, scan_guides, rows, gid, below, inp_of_pair) -> pd.DataFrame:
@@ -559,8 +596,10 @@ class beguidedoff:
         n_ids = int(gid.max()) + 1 if gid.size else 0
         per_id = np.bincount(rid, minlength=n_ids)
         starts = np.cumsum(per_id) - per_id
-        sel = np.concatenate([np.arange(starts[gid[i]], starts[gid[i]] + per_id[gid[i]])
-                              for i in range(len(guides))])
+        # each input guide's block of its id's rows, in input order
+        n_in = per_id[gid]
+        first = np.cumsum(n_in) - n_in
+        sel = np.repeat(starts[gid] - first, n_in) + np.arange(int(n_in.sum()))
         if sel.size == 0:
             return pd.DataFrame()
         r = rows[sel]
@@ -597,14 +636,16 @@ class beguidedoff:
             "ot_sequence": _strings(codes[:, :20]),
             "ot_pam": pam_str,
             "n_mismatches": h["n_mm"].astype(np.int64),
-            "mismatch_positions": np.array([_mm_string(int(m)) for m in h["mm_mask"]],
+            "mismatch_positions": np.array([_mm_string(m) for m in h["mm_mask"].tolist()],
                                            dtype=object),
-            "score": np.array([_round6(float(x)) for x in h["score"]], dtype=np.float64),
+            "score": np.array([_round6(x) for x in h["score"].tolist()], dtype=np.float64),
             "n_offtargets_below_threshold": below[gid[inp]],
         }
         return pd.DataFrame(cols, columns=OUTPUT_COLUMNS)
 
     def close(self) -> None:
+        with self._table_lock:
+            self._close_table_plan()
         if self._genome_handle is not None:
             self._genome_handle.close()
             self._genome_handle = None
@@ -640,8 +681,10 @@ def _strings(codes: np.ndarray) -> np.ndarray:
     n, w = codes.shape
     if n == 0:
         return np.zeros(0, dtype=object)
-    b = _ACGT[codes].reshape(n * w).tobytes()
-    return np.array([b[i * w:(i + 1) * w].decode("ascii") for i in range(n)], dtype=object)
+    b = _ACGT[codes].tobytes().decode("ascii")
+    out = np.empty(n, dtype=object)
+    out[:] = [b[i:i + w] for i in range(0, n * w, w)]
+    return out
 
 
 _MM_STR_CACHE: Dict[int, str] = {}

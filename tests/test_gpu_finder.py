@@ -273,3 +273,19 @@ def test_seed_engine_tables_match_oracle(gpu, seed):
         got = f.search(guides=guides, verbose=False, **kw)
         pd.testing.assert_frame_equal(got, want)
     assert len(want) > 5
+
+
+def test_searches_in_a_row_reuse_the_plan(gpu):
+    """The batched finder keeps its device-table plan between searches: on
+    one finder, searches with other guide sets, orders and options each give
+    the oracle's table, and close() releases the plan."""
+    genome, guides = random_case(1)
+    f = finder(genome, "batched", max_mismatches=5)
+    oracle = FinderOracle(genome, max_mismatches=5)
+    for sub in (guides, guides[: max(1, len(guides) // 3)], guides[::-1], guides):
+        for kw in ({}, {"return_all": True, "count_score_threshold": 5.0}):
+            got = f.search(guides=sub, verbose=False, **kw)
+            pd.testing.assert_frame_equal(got, oracle.search(sub, **kw))
+    assert f._table_plan is not None
+    f.close()
+    assert f._table_plan is None
