@@ -96,9 +96,9 @@ TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_602_112_133,
 
 # Busy fractions of the units of the dominant kernel from rocprofv3 --pmc
 # (VALU: 4 cycles per SQ_INSTS_VALU instruction; MFMA: SQ_VALU_MFMA_BUSY_CYCLES;
-# both over GRBM_GUI_ACTIVE SIMD-cycles) on the same workload, same profile as
-# the traffic.
-UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.680, "mfma": 0.512}}
+# both together: SQ_VALU_MFMA_COEXEC_CYCLES; all over GRBM_GUI_ACTIVE
+# SIMD-cycles) on the same workload, same profile as the traffic.
+UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.680, "mfma": 0.512, "valu_and_mfma": 0.226}}
 
 # The seed engine on 8192 NGG guides (two 4,096-guide tables, two launches
 # per step): VALU busy from SQ_INSTS_VALU over GRBM_GUI_ACTIVE SIMD-cycles.
@@ -133,11 +133,15 @@ def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, f
            "survey_ops_per_launch": int(survey_ops),
            "hbm_gbps_planes": round(2 * L / 4 / secs / 1e9, 1)}
     if engine >= 2 and busy:
-        top = max(busy, key=busy.get)
+        top = max(("valu", "mfma"), key=busy.get)
+        both = busy.get("valu_and_mfma")
         out["binding_unit"] = (
-            f"{top.upper()} {busy[top]:.0%} busy, none saturated: the scan and append phases "
-            "are chains of dependent VALU/LDS steps at 4 waves per SIMD, and the MFMA pipe "
-            "idles while a wave runs them" if busy[top] < 0.8 else f"{top.upper()}")
+            f"{top.upper()} {busy[top]:.0%} busy, none saturated: the VALU and MFMA phases "
+            "overlap little" + (f" (both busy in {both:.0%} of SIMD-cycles, neither in "
+                                f"{1 - busy['valu'] - busy['mfma'] + both:.0%})" if both else "")
+            + "; the scan and append phases are chains of dependent VALU/LDS steps at 4 "
+            "waves per SIMD, and a wave's compare waits for its products before testing them"
+            if busy[top] < 0.8 else f"{top.upper()}")
         out["unit_busy"] = dict(busy, source=traffic[1] if traffic else None)
     seed_busy = SEED_BUSY.get((workload, engine, max_mm))
     if engine == 4 and seed_busy:
