@@ -584,3 +584,32 @@ def test_seed_engine_needs_budgets_up_to_4(gpu):
     dg = DeviceGenome(genome)
     with pytest.raises(RuntimeError, match="budgets <= 4"):
         scan(dg, [GuideSpec(s, p, 5) for s, p in guides], tuning=dict(SEED))
+
+
+@pytest.mark.parametrize("engine", [pytest.param({"engine": e}, id=str(e)) for e in (1, 2, 3, 4)])
+@pytest.mark.parametrize("seed", [3, 7])
+def test_strand_symmetry_and_guide_order(gpu, engine, seed):
+    """Two invariants of the scan: the − strand sites of a genome are the +
+    strand sites of the genome built from its contigs' reverse complements
+    (same contig order: the reverse-complement array is reversed per contig),
+    and a permuted guide order gives every guide the same sites."""
+    from beguidedoff_amd.encoding import reverse_complement_seq
+
+    contigs, guides, max_mm = build_case(seed, length=(3_000, 9_000), n_guides=6)
+    g1 = synth.genome_from_strings(contigs)
+    g2 = synth.genome_from_strings([(n, reverse_complement_seq(s)) for n, s in contigs])
+    specs = [GuideSpec(s, p, max_mm) for s, p in guides]
+    d1 = DeviceGenome(g1, g1["n_mask_le_fwd"], g1["n_mask_le_rc"])
+    d2 = DeviceGenome(g2, g2["n_mask_le_fwd"], g2["n_mask_le_rc"])
+    try:
+        minus = {(g, p, m) for g, s, p, m in hit_set(scan(d1, specs, tuning=engine).hits) if s == 1}
+        plus = {(g, p, m) for g, s, p, m in hit_set(scan(d2, specs, tuning=engine,
+                                                          strands=1).hits)}
+        assert minus == plus and len(minus) > 0
+        perm = np.random.default_rng(seed).permutation(len(specs))
+        got = {(int(perm[g]), s, p, m)
+               for g, s, p, m in hit_set(scan(d1, [specs[i] for i in perm], tuning=engine).hits)}
+        assert got == hit_set(scan(d1, specs, tuning=engine).hits)
+    finally:
+        d1.close()
+        d2.close()
