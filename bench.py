@@ -50,14 +50,15 @@ def device_peaks(info: dict) -> dict:
 
 
 def algorithmic_ops(L_pos: int, n_strands: int, families) -> tuple:
-    """(ops of the bit-plane formulation, ops of the per-unit figures used in
-    the reference survey) for one launch; families = [(guides, windows)]
-    where windows counts PAM-passing windows over both strands."""
+    """(ops of the bit-plane formulation, ops of a per-unit cost model of a
+    plain INT32 compare: 3 per position and family, 11 per (guide, window)
+    pair) for one launch; families = [(guides, windows)] where windows counts
+    PAM-passing windows over both strands."""
     ops = n_strands * L_pos * 2                       # window + PAM extraction
     ops += sum(n_strands * L_pos * 4 for _ in families)  # 3 bit-selects + compare
     ops += sum(4 * g * w for g, w in families)        # xor, bitop3, bcnt, test
-    survey = sum(n_strands * L_pos * 3 + 11 * g * w for g, w in families)
-    return ops, survey
+    model = sum(n_strands * L_pos * 3 + 11 * g * w for g, w in families)
+    return ops, model
 
 
 ENGINE_NAMES = {1: "batched VALU", 2: "batched int8 MFMA", 3: "batched fp4 MFMA",
@@ -115,9 +116,9 @@ def site_keys(guide, strand, pos, n_mm) -> np.ndarray:
 
 
 def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, families,
-                   survey_ops, L, kms, peaks) -> dict:
+                   model_ops, L, kms, peaks) -> dict:
     """The dominant kernel against its peak, plus what binds it and how the
-    per-unit models of the survey price the same launch."""
+    per-unit cost models of a plain compare price the same launch."""
     traffic = TRAFFIC_BYTES.get((workload, engine, max_mm))
     busy = UNIT_BUSY.get((workload, engine, max_mm))
     pairs = int(sum(gg * ww for gg, ww in families))
@@ -130,7 +131,7 @@ def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, f
            "kernel": "seed_scan_kernel" if engine == 4 else "batched_scan_kernel",
            "ops_per_launch": int(alg),
            "pairs_per_launch": pairs,
-           "survey_ops_per_launch": int(survey_ops),
+           "unit_model_ops_per_launch": int(model_ops),
            "hbm_gbps_planes": round(2 * L / 4 / secs / 1e9, 1)}
     if engine >= 2 and busy:
         top = max(("valu", "mfma"), key=busy.get)
@@ -150,15 +151,15 @@ def roofline_block(bound, achieved, peak, unit, workload, engine, max_mm, alg, f
             "VALU lane-slots: a wave walks its 64 windows' guide lists for as long as the "
             "longest of them")
         out["unit_busy"] = dict(seed_busy[0], source=seed_busy[1])
-    # The survey's per-unit models of the same launch: 11 INT32 VALU ops per
+    # Per-unit cost models of the same launch: 11 INT32 VALU ops per
     # (guide, window) pair + 3 per window and family, against the INT32 VALU
     # peak; 160 int8 ops per pair against the dense int8 MFMA peak.  The
     # bit-plane and MFMA formulations do less work per pair than the first
     # model assumes, so a frac above 1 there means the model no longer bounds
     # this engine, not that a peak was exceeded.
-    out["survey_models"] = {
-        "int32_valu": {"ops": int(survey_ops), "peak_tops": round(valu_peak / 2 / 1e12, 1),
-                       "frac": round(survey_ops / secs / (valu_peak / 2), 3)},
+    out["unit_models"] = {
+        "int32_valu": {"ops": int(model_ops), "peak_tops": round(valu_peak / 2 / 1e12, 1),
+                       "frac": round(model_ops / secs / (valu_peak / 2), 3)},
         "int8_onehot_mfma": {"ops": 160 * pairs, "peak_tops": 5000.0,
                              "frac": round(160 * pairs / secs / 5.0e15, 3)}}
     return out
@@ -580,9 +581,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "the scan runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
-        print("bench.py: only --gpus 1 is supported (one MI355X per box)", file=sys.stderr)
+        print("bench.py: only --gpus 1 is supported (one MI355X; how the scan would shard is a design note)", file=sys.stderr)
         return 2
 
     if args.copy_probe > 0:
@@ -669,7 +670,7 @@ def main() -> int:
     kms = float(np.mean(kernel_ms))
     families = plan.family_stats()
     L_pos = max(0, L - 26)
-    ops, survey_ops = algorithmic_ops(L_pos, 2, families)
+    ops, model_ops = algorithmic_ops(L_pos, 2, families)
     engine = plan.tuning()["engine"]
     if engine == 2:  # int8 MFMA compare: priced against the dense int8 MFMA peak
         bound, unit, dtype = "mfma", "TOP/s", "int (exact); int8 MFMA operands"
@@ -728,7 +729,7 @@ def main() -> int:
         "autotune": tune_trials,
         "kernel_ms": round(kms, 3),
         "roofline": roofline_block(bound, achieved, peak, unit, w.name, engine, args.max_mm,
-                                   alg, families, survey_ops, L, kms, peaks),
+                                   alg, families, model_ops, L, kms, peaks),
     }
 
     # the batched site set, as a hash of its sorted (guide, strand, pos, n_mm) keys
