@@ -663,8 +663,9 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
             b1 = btile(ti + 4);
             b2 = btile(ti + 5);
           }
+          // (no barrier here: the compiler then issues this pair's field
+          // tests between the next pair's products)
           any |= or16(c) | or16(d);
-          asm volatile("" : "+v"(any));
         }
         if (ballot((any & kField3Mask) != 0)) {
           // Rare: the guide tiles whose field has a set bit somewhere in the
