@@ -92,14 +92,14 @@ def mfma_ops(families) -> int:
 # step loop, each scaled by the same counter's reading for a copy kernel of
 # known size.  Keyed by (workload, engine, max_mm): (bytes, the profile that
 # holds the counter runs and their commands).  None when not profiled.
-TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_602_112_133,
+TRAFFIC_BYTES = {("hg38_mixed", 3, 4): (1_601_989_374,
                                         "profiles/batched_scan_kernel_fp4_hg38_mixed_1024g_4mm.md")}
 
 # Busy fractions of the units of the dominant kernel from rocprofv3 --pmc
 # (VALU: 4 cycles per SQ_INSTS_VALU instruction; MFMA: SQ_VALU_MFMA_BUSY_CYCLES;
 # both together: SQ_VALU_MFMA_COEXEC_CYCLES; all over GRBM_GUI_ACTIVE
 # SIMD-cycles) on the same workload, same profile as the traffic.
-UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.680, "mfma": 0.512, "valu_and_mfma": 0.226}}
+UNIT_BUSY = {("hg38_mixed", 3, 4): {"valu": 0.681, "mfma": 0.517, "valu_and_mfma": 0.232}}
 
 # The seed engine on 8192 NGG guides (two 4,096-guide tables, two launches
 # per step): VALU busy from SQ_INSTS_VALU over GRBM_GUI_ACTIVE SIMD-cycles.
