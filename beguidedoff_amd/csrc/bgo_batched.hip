@@ -249,7 +249,6 @@ __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, c
   if (lane >= n) return;
   const uint2 e = L[lane];
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
-  const uint32_t pos = entry_pos(t.runs, e), s = entry_strand(t.runs, e);
   const uint32_t kmax = F.kmax;
   const uint32_t* __restrict__ g01 = t.g01;
   for (uint32_t g = F.g_begin; g < F.g_end; g += kUnroll) {
@@ -273,7 +272,11 @@ __device__ inline void compare_list(const Args& a, const Tab& t, const Fam& F, c
         const uint32_t gi = g + i;
         const uint32_t ci = __popc((W0 ^ g01[2 * gi]) | (W1 ^ g01[2 * gi + 1]));
         const bool cand = static_cast<int>(ci) <= t.gk[gi];
-        if (ballot(cand)) record_candidates(a, t, s, gi, pos, W0, W1, ci, cand, lane);
+        // the window's position and strand only here (they cost a read of
+        // the run slots)
+        if (ballot(cand))
+          record_candidates(a, t, entry_strand(t.runs, e), gi, entry_pos(t.runs, e), W0, W1, ci,
+                            cand, lane);
       }
     }
   }
@@ -754,9 +757,8 @@ __host__ __device__ inline uint32_t seed_other(uint32_t w, uint32_t j) {
 // of the window's key in block j), its full 2-bit count, and the pair only if
 // block j is the first block in which it matches exactly, so a pair matching
 // in several blocks is recorded once.
-__device__ inline void seed_exact(const Args& a, const Tab& t, uint32_t s, uint32_t lane,
-                                  uint32_t W0, uint32_t W1, uint32_t pos, uint32_t j,
-                                  uint32_t widx, bool cand) {
+__device__ inline void seed_exact(const Args& a, const Tab& t, uint2 e, uint32_t lane,
+                                  uint32_t W0, uint32_t W1, uint32_t j, uint32_t widx, bool cand) {
   uint32_t gi = 0, c = 0;
   bool rec = false;
   if (cand) {
@@ -769,7 +771,9 @@ __device__ inline void seed_exact(const Args& a, const Tab& t, uint32_t s, uint3
     c = static_cast<uint32_t>(__popc(m));
     rec = (nz & upto) == below && static_cast<int>(c) <= t.gk[gi];
   }
-  if (ballot(rec)) record_candidates(a, t, s, gi, pos, W0, W1, c, rec, lane);
+  if (ballot(rec))
+    record_candidates(a, t, entry_strand(t.runs, e), gi, entry_pos(t.runs, e), W0, W1, c, rec,
+                      lane);
 }
 
 // Compare the first n (<= 64) windows of a family ring with the family's
@@ -781,7 +785,6 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, c
   const bool has = lane < n;
   const uint2 e = has ? L[lane] : make_uint2(0u, 0u);
   const uint32_t W0 = e.x & kWin, W1 = e.y & kWin;
-  const uint32_t pos = entry_pos(t.runs, e), s = entry_strand(t.runs, e);
   const uint32_t kmax = F.kmax;
   const uint32_t* st = t.stab + F.seed_start;
   const uint32_t* en = t.stab + F.seed_ent;
@@ -833,7 +836,7 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, c
 #pragma nounroll
         for (uint32_t q = 0; q < kSeedStep; ++q) {
           const bool cand = p[q] <= kmax && i + q < end;
-          if (ballot(cand)) seed_exact(a, t, s, lane, W0, W1, pos, j, F.seed_ent + b + q, cand);
+          if (ballot(cand)) seed_exact(a, t, e, lane, W0, W1, j, F.seed_ent + b + q, cand);
         }
       }
     }
