@@ -308,14 +308,16 @@ __device__ inline uint32_t direct_passes(const Args& a, const Tab& t, const Fam&
   uint32_t n = 0;
   for (uint64_t act = ballot(pw != 0); act; act = ballot(pw != 0)) {
     n += static_cast<uint32_t>(__popcll(act));
-    const bool has = pw != 0;
-    const uint32_t j = has ? static_cast<uint32_t>(__builtin_ctz(pw)) : 0u;
+    // A lane without a window reads the one at bit 31 (pw | bit 31 is never
+    // empty); the exact path below records windows of lanes with pw != 0 only.
+    const uint32_t j = static_cast<uint32_t>(__builtin_ctz(pw | 0x80000000u));
     const uint32_t W0 = funnel(qb.x, qa.x, j) & kWin, W1 = funnel(qb.y, qa.y, j) & kWin;
     uint32_t m = __popc((W0 ^ gx[0]) | (W1 ^ gy[0]));
 #pragma unroll
     for (int i = 1; i < NG; ++i)
       m = min(m, static_cast<uint32_t>(__popc((W0 ^ gx[i]) | (W1 ^ gy[i]))));
-    if (ballot(has && m <= kmax)) {  // rare: per-guide budgets and the exact path
+    if (ballot(m <= kmax)) {  // rare: per-guide budgets and the exact path
+      const bool has = pw != 0;
       const uint32_t pos = (bb << 5) + j;
 #pragma nounroll
       for (uint32_t i = 0; i < NG; ++i) {
