@@ -808,8 +808,8 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, c
     const uint32_t beg = s0 & ~1u;
     const uint32_t end = has ? (s1 & ~1u) - (s0 & 1u) : beg;
     verified += end - beg;
-    // kSeedStep entries (word pairs of two) per step, read together.  A lane whose
-    // key's entries are done goes on reading the following ones while the
+    // kSeedStep entries (word pairs of two) per step, read together.  A lane
+    // whose key's entries are done goes on reading the following ones while the
     // wave's longest walk lasts; only the rare candidates are checked to lie
     // in the lane's own key (i + q < end), and the reads stay in the
     // family's table (clamped to its end, which is followed by kSeedPad
@@ -825,8 +825,8 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, c
       uint32_t lo = 32;
 #pragma unroll
       for (uint32_t q = 0; q < kSeedStep / 2; ++q) {
-        const uint2 e = pb[q];
-        const uint32_t m = (e.x ^ wl) | (e.y ^ wh);
+        const uint2 pr = pb[q];  // entries 2q, 2q + 1 of the step
+        const uint32_t m = (pr.x ^ wl) | (pr.y ^ wh);
         const uint32_t h = static_cast<uint32_t>(__popc(m >> 16));
         p[2 * q] = static_cast<uint32_t>(__popc(m)) - h;
         p[2 * q + 1] = h;
