@@ -192,7 +192,9 @@ def test_config3_5mm_oracle_slices(mixed, mixed_5mm):
 
 def test_config3_5mm_whole_genome_oracle_subset(mixed):
     """Two guides of each PAM family and two A-rich guides, <= 5 mm, over the
-    whole genome: the oracle equals the plain kernel and all three engines."""
+    whole genome: the oracle equals the plain kernel and the VALU, int8 and fp4
+    engines; at <= 4 mm, the oracle's sites with n_mm <= 4 equal every engine's,
+    the seed engine's included."""
     w, dg = mixed
     g = w.genome
     pick = []
@@ -207,6 +209,10 @@ def test_config3_5mm_whole_genome_oracle_subset(mixed):
     for e in (1, 2, 3):
         assert hit_set(scan(dg, specs, tuning={"engine": e}).hits) == want, e
     assert plain_set(g, guides, 5, rpg=200_000) == want
+    want4 = {h for h in want if h[3] <= 4}
+    specs4 = [GuideSpec(s, p, 4) for s, p in guides]
+    for e in (1, 3, 4):
+        assert hit_set(scan(dg, specs4, tuning={"engine": e}).hits) == want4, e
 
 
 def test_config3_5mm_plants_found(gpu):
