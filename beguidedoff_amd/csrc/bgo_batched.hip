@@ -27,6 +27,7 @@
 // budget, fused epilogue and a wave-aggregated append of hit records.
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <set>
@@ -747,7 +748,7 @@ __device__ inline void compare_list_mfma(const Args& a, const Tab& t, const Fam&
 // budgets are all <= 4 use it.
 constexpr uint32_t kSeedBlocks = 5;
 constexpr uint32_t kSeedKeys = 256;
-constexpr uint32_t kSeedStep = 8;  // table entries (word pairs x 2) read per step of a walk
+constexpr uint32_t kSeedStep = 8;  // most table entries (word pairs x 2) read per step of a walk
 constexpr uint32_t kSeedPad = kSeedStep;  // spare words after a family's entries
 
 // The 16 bits of a 20-bit plane word outside bases 4j..4j+3, packed in order.
@@ -778,9 +779,53 @@ __device__ inline void seed_exact(const Args& a, const Tab& t, uint2 e, uint32_t
                       lane);
 }
 
+// One window's walk through the guides of its key in block j (entries
+// [beg, end) of the family's table), STEP entries (word pairs of two) per
+// step, read together.  A lane whose key's entries are done goes on reading
+// the following ones while the wave's longest walk lasts; only the rare
+// candidates are checked to lie in the lane's own key (i + q < end), and the
+// reads stay in the family's table (clamped to its end, which is followed by
+// kSeedPad spare words; a clamped read is never in the lane's key).
+template <uint32_t STEP>
+__device__ inline void seed_walk(const Args& a, const Tab& t, const Fam& F, uint2 e, uint32_t lane,
+                                 uint32_t W0, uint32_t W1, uint32_t j, uint32_t wl, uint32_t wh,
+                                 uint32_t beg, uint32_t end, const uint32_t* en, uint32_t last,
+                                 uint32_t kmax) {
+  static_assert(STEP % 2 == 0 && STEP <= kSeedPad, "steps of word pairs within the spare words");
+  const uint2* pend = reinterpret_cast<const uint2*>(en + end);
+  const uint2* plast = reinterpret_cast<const uint2*>(en + last);
+  for (const uint2* pi = reinterpret_cast<const uint2*>(en + beg); ballot(pi < pend);
+       pi += STEP / 2) {
+    const uint2* pb = pi < plast ? pi : plast;  // entries 2 (pb - en) .. (even)
+    // per word pair, each half's count: the high half's, and the whole
+    // word's less it
+    uint32_t p[STEP];
+    uint32_t lo = 32;
+#pragma unroll
+    for (uint32_t q = 0; q < STEP / 2; ++q) {
+      const uint2 pr = pb[q];  // entries 2q, 2q + 1 of the step
+      const uint32_t m = (pr.x ^ wl) | (pr.y ^ wh);
+      const uint32_t h = static_cast<uint32_t>(__popc(m >> 16));
+      p[2 * q] = static_cast<uint32_t>(__popc(m)) - h;
+      p[2 * q + 1] = h;
+      lo = min(lo, min(p[2 * q], h));
+    }
+    if (ballot(lo <= kmax)) {
+      const uint32_t b = static_cast<uint32_t>(reinterpret_cast<const uint32_t*>(pb) - en);
+      const uint32_t i = static_cast<uint32_t>(reinterpret_cast<const uint32_t*>(pi) - en);
+#pragma nounroll
+      for (uint32_t q = 0; q < STEP; ++q) {
+        const bool cand = p[q] <= kmax && i + q < end;
+        if (ballot(cand)) seed_exact(a, t, e, lane, W0, W1, j, F.seed_ent + b + q, cand);
+      }
+    }
+  }
+}
+
 // Compare the first n (<= 64) windows of a family ring with the family's
 // guides through the seed table: one window per lane; per block, each lane
 // walks its own key's guides, the wave as long as its longest walk.
+template <uint32_t STEP>
 __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, const uint2* L,
                                     uint32_t n, uint32_t lane, uint32_t* vcnt) {
   uint32_t verified = 0;
@@ -808,40 +853,7 @@ __device__ inline void seed_compare(const Args& a, const Tab& t, const Fam& F, c
     const uint32_t beg = s0 & ~1u;
     const uint32_t end = has ? (s1 & ~1u) - (s0 & 1u) : beg;
     verified += end - beg;
-    // kSeedStep entries (word pairs of two) per step, read together.  A lane
-    // whose key's entries are done goes on reading the following ones while the
-    // wave's longest walk lasts; only the rare candidates are checked to lie
-    // in the lane's own key (i + q < end), and the reads stay in the
-    // family's table (clamped to its end, which is followed by kSeedPad
-    // spare words; a clamped read is never in the lane's key).
-    const uint2* pend = reinterpret_cast<const uint2*>(en + end);
-    const uint2* plast = reinterpret_cast<const uint2*>(en + last);
-    for (const uint2* pi = reinterpret_cast<const uint2*>(en + beg); ballot(pi < pend);
-         pi += kSeedStep / 2) {
-      const uint2* pb = pi < plast ? pi : plast;  // entries 2 (pb - en) .. (even)
-      // per word pair, each half's count: the high half's, and the whole
-      // word's less it
-      uint32_t p[kSeedStep];
-      uint32_t lo = 32;
-#pragma unroll
-      for (uint32_t q = 0; q < kSeedStep / 2; ++q) {
-        const uint2 pr = pb[q];  // entries 2q, 2q + 1 of the step
-        const uint32_t m = (pr.x ^ wl) | (pr.y ^ wh);
-        const uint32_t h = static_cast<uint32_t>(__popc(m >> 16));
-        p[2 * q] = static_cast<uint32_t>(__popc(m)) - h;
-        p[2 * q + 1] = h;
-        lo = min(lo, min(p[2 * q], h));
-      }
-      if (ballot(lo <= kmax)) {
-        const uint32_t b = static_cast<uint32_t>(reinterpret_cast<const uint32_t*>(pb) - en);
-        const uint32_t i = static_cast<uint32_t>(reinterpret_cast<const uint32_t*>(pi) - en);
-#pragma nounroll
-        for (uint32_t q = 0; q < kSeedStep; ++q) {
-          const bool cand = p[q] <= kmax && i + q < end;
-          if (ballot(cand)) seed_exact(a, t, e, lane, W0, W1, j, F.seed_ent + b + q, cand);
-        }
-      }
-    }
+    seed_walk<STEP>(a, t, F, e, lane, W0, W1, j, wl, wh, beg, end, en, last, kmax);
   }
   vcnt[lane] += verified;
 }
@@ -863,7 +875,7 @@ struct WaveState {
 
 // Compare the batch at the front of family f's buffer (ring_flush windows)
 // and move the at most 64 windows behind it to the front.
-template <int ENGINE>
+template <int ENGINE, int SS = 8>
 __device__ inline void flush_front(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
                                    uint2* L, uint32_t lane) {
   constexpr uint32_t kFlush = ring_flush(ENGINE);
@@ -873,7 +885,7 @@ __device__ inline void flush_front(const Args& a, const Tab& t, const WaveState&
   else if constexpr (ENGINE == 4)
   {
     __builtin_amdgcn_s_setprio(0);
-    seed_compare(a, t, t.fam[f], L, kFlush, lane, w.vcnt + f * 64);
+    seed_compare<SS>(a, t, t.fam[f], L, kFlush, lane, w.vcnt + f * 64);
     __builtin_amdgcn_s_setprio(1);
   }
   else
@@ -894,7 +906,7 @@ __device__ inline void flush_front(const Args& a, const Tab& t, const WaveState&
 // this.  Each step appends the next window of every lane that has one (slots
 // by ballot + mbcnt), so all lanes write in parallel; the ring always keeps
 // room for one such step.
-template <int ENGINE>
+template <int ENGINE, int SS = 8>
 __device__ inline void append_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
                                      uint32_t pw, uint32_t bb, uint2 qa, uint2 qb, uint32_t cb,
                                      uint32_t flag, uint32_t lane) {
@@ -915,7 +927,7 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
   uint64_t act = ballot(pw != 0);
   while (act) {
     if (cnt > kSize - 64) {
-      flush_front<ENGINE>(a, t, w, f, L, lane);
+      flush_front<ENGINE, SS>(a, t, w, f, L, lane);
       cnt -= kFlush;
       old = old > kFlush ? old - kFlush : 0u;
       continue;
@@ -944,12 +956,12 @@ __device__ inline void append_family(const Args& a, const Tab& t, const WaveStat
 // to the family's ring in LDS; every ring_flush(ENGINE) buffered windows are
 // compared against the family's guides.
 // Compare every window in family f's buffer and empty it.
-template <int ENGINE>
+template <int ENGINE, int SS = 8>
 __device__ inline void drain_family(const Args& a, const Tab& t, const WaveState& w, uint32_t f,
                                     uint32_t lane) {
   uint32_t c = uniform(w.counts[f]) & 0xFFFFu;
   if (c > ring_flush(ENGINE)) {
-    flush_front<ENGINE>(a, t, w, f, w.ring<ENGINE>(f), lane);
+    flush_front<ENGINE, SS>(a, t, w, f, w.ring<ENGINE>(f), lane);
     c -= ring_flush(ENGINE);
   }
   if (c) {
@@ -959,7 +971,7 @@ __device__ inline void drain_family(const Args& a, const Tab& t, const WaveState
     else if constexpr (ENGINE == 4)
     {
       __builtin_amdgcn_s_setprio(0);
-      seed_compare(a, t, t.fam[f], w.ring<ENGINE>(f), c, lane, w.vcnt + f * 64);
+      seed_compare<SS>(a, t, t.fam[f], w.ring<ENGINE>(f), c, lane, w.vcnt + f * 64);
       __builtin_amdgcn_s_setprio(1);
     }
     else
@@ -975,7 +987,7 @@ __device__ inline void drain_family(const Args& a, const Tab& t, const WaveState
   wave_sync();
 }
 
-template <int NF, int ENGINE>
+template <int NF, int ENGINE, int SS = 8>
 __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w, uint32_t s,
                                 uint32_t cs, uint32_t ce, uint32_t flag, uint32_t lane) {
   const uint2* __restrict__ planes = s ? a.planes1 : a.planes0;
@@ -1058,7 +1070,7 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
           const uint32_t n = direct_family(a, t, t.fam[f], s, pw, bb, qa, qb, lane);
           if (lane == 0) w.wstats[f] += n;
         } else {
-          append_family<ENGINE>(a, t, w, f, pw, bb, qa, qb, cb, flag, lane);
+          append_family<ENGINE, SS>(a, t, w, f, pw, bb, qa, qb, cb, flag, lane);
         }
       }
     }
@@ -1068,11 +1080,11 @@ __device__ inline void scan_run(const Args& a, const Tab& t, const WaveState& w,
   // into the wave's next run, where they fill whole batches.
 #pragma nounroll
   for (uint32_t f = 0; f < NF; ++f)
-    if (uniform(w.counts[f]) >> 16) drain_family<ENGINE>(a, t, w, f, lane);
+    if (uniform(w.counts[f]) >> 16) drain_family<ENGINE, SS>(a, t, w, f, lane);
   wave_sync();
 }
 
-template <int NF, int ENGINE>
+template <int NF, int ENGINE, int SS = 8>
 __device__ __forceinline__ void scan_kernel_body(
     uint2* lds, Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
     const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
@@ -1171,13 +1183,13 @@ __device__ __forceinline__ void scan_kernel_body(
     const uint32_t s = slot ? a.slot_strand1 : a.slot_strand0;
     const uint32_t cs = a.valid_begin + run * a.chunk;
     const uint32_t ce = min(cs + a.chunk, a.valid_end);
-    scan_run<NF, ENGINE>(a, t, w, s, cs, ce, n_run & 1u, lane);
+    scan_run<NF, ENGINE, SS>(a, t, w, s, cs, ce, n_run & 1u, lane);
     ++n_run;
   }
   // the windows the last runs left in the buffers
 #pragma nounroll
   for (uint32_t f = 0; f < NF; ++f)
-    if (uniform(w.counts[f]) & 0xFFFFu) drain_family<ENGINE>(a, t, w, f, lane);
+    if (uniform(w.counts[f]) & 0xFFFFu) drain_family<ENGINE, SS>(a, t, w, f, lane);
   if constexpr (ENGINE >= 2) flush_records(a, t, lane);
   wave_sync();
   if (lane < NF && w.wstats[lane]) atomicAdd(&fam_windows[lane], w.wstats[lane]);
@@ -1209,7 +1221,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(4))) 
 // The seed engine's kernel: one block of kSeedBlock threads per CU shares the
 // seed table in LDS.
 constexpr int kSeedBlock = 1024;
-template <int NF>
+template <int NF, int SS>
 __global__ __launch_bounds__(kSeedBlock) void seed_scan_kernel(
     Args a, const Fam* __restrict__ fam, const uint32_t* __restrict__ g01,
     const int32_t* __restrict__ gk, const uint32_t* __restrict__ gaux,
@@ -1218,8 +1230,8 @@ __global__ __launch_bounds__(kSeedBlock) void seed_scan_kernel(
     unsigned long long* __restrict__ hit_count, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ fam_windows, const int4* __restrict__ btab) {
   extern __shared__ uint2 lds[];
-  scan_kernel_body<NF, 4>(lds, a, fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count,
-                          work_counter, fam_windows, btab);
+  scan_kernel_body<NF, 4, SS>(lds, a, fam, g01, gk, gaux, gid, gnm, score_tab, hits, hit_count,
+                              work_counter, fam_windows, btab);
 }
 
 using KernelFn = void (*)(Args, const Fam*, const uint32_t*, const int32_t*, const uint32_t*,
@@ -1244,15 +1256,19 @@ inline KernelFn kernel_for_nf(int nf) {
 // the shared table share one block's LDS).
 constexpr int kSeedMaxFam = 4;
 
-inline KernelFn kernel_for(int nf, int engine) {
-  if (engine == 4) {
-    switch (nf) {
-      case 1: return seed_scan_kernel<1>;
-      case 2: return seed_scan_kernel<2>;
-      case 3: return seed_scan_kernel<3>;
-      default: return seed_scan_kernel<4>;
-    }
+template <int SS>
+inline KernelFn seed_kernel_for(int nf) {
+  switch (nf) {
+    case 1: return seed_scan_kernel<1, SS>;
+    case 2: return seed_scan_kernel<2, SS>;
+    case 3: return seed_scan_kernel<3, SS>;
+    default: return seed_scan_kernel<4, SS>;
   }
+}
+
+// seed_step: table entries per step of the seed engine's walks (4 or 8).
+inline KernelFn kernel_for(int nf, int engine, int seed_step = 8) {
+  if (engine == 4) return seed_step == 4 ? seed_kernel_for<4>(nf) : seed_kernel_for<8>(nf);
   return engine == 3 ? kernel_for_nf<3>(nf) : engine == 2 ? kernel_for_nf<2>(nf) : kernel_for_nf<1>(nf);
 }
 
@@ -1325,6 +1341,7 @@ struct bgo_plan {
   std::vector<size_t> group_lds;     // dynamic LDS bytes per launch group
   std::vector<size_t> group_img_off;    // seed engine: byte offset of each group's table in d_btab
   std::vector<size_t> group_img_bytes;  // seed engine: bytes of each group's table (LDS copy)
+  std::vector<int> group_step;          // seed engine: table entries per walk step (4 or 8)
   int threads = 0;
   // Tables and hit buffer in memory the plan does not own (the no-alloc
   // entry point's scratch array): never grown, never freed.
@@ -1511,13 +1528,14 @@ int plan_configure_launch(bgo_plan* P) {
     const size_t per_wave = bk::wave_lds_bytes(nf, P->tune.engine);
     const size_t lds = seed ? align256(P->group_img_bytes[gi]) + per_wave * (bk::kSeedBlock / 64)
                             : per_wave * P->tune.waves_per_block;
+    const int step = seed ? P->group_step[gi] : 8;
     if (seed)
-      BGO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(bk::kernel_for(nf, 4)),
+      BGO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(bk::kernel_for(nf, 4, step)),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(lds)));
     int occ = 0;
-    BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bk::kernel_for(nf, P->tune.engine), P->threads,
-                                                         lds));
+    BGO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, bk::kernel_for(nf, P->tune.engine, step), P->threads, lds));
     if (occ <= 0) occ = 1;
     if (P->tune.blocks_per_cu > 0) occ = std::min(occ, P->tune.blocks_per_cu);
     P->n_cu = n_cu;
@@ -1663,12 +1681,37 @@ std::vector<int> seed_groups(const std::vector<uint32_t>& part_guides) {
   return first;
 }
 
+// Expected VALU of a wave's walks through one block's key lists (sizes padded
+// to even) in steps of `step` entries: the wave walks as long as the longest
+// list among its 64 windows, E[max] = sum over t >= 0 of 1 - P(steps <= t)^64,
+// with the window keys drawn with the base odds of a 41% GC genome; a step of
+// 4 entries issues 18 VALU, a step of 8 entries 32.
+double seed_walk_cost(const std::vector<uint32_t>& sizes, uint32_t step) {
+  static const double kBase[4] = {0.295, 0.205, 0.205, 0.295};  // A C G T
+  std::vector<double> p_steps;
+  for (uint32_t k = 0; k < bk::kSeedKeys; ++k) {
+    double pk = 1.0;
+    for (uint32_t b = 0; b < 4; ++b) pk *= kBase[((k >> b) & 1u) | (((k >> (4 + b)) & 1u) << 1)];
+    const uint32_t n = (sizes[k] + step - 1) / step;
+    if (p_steps.size() <= n) p_steps.resize(n + 1, 0.0);
+    p_steps[n] += pk;
+  }
+  double e = 0.0, cdf = 0.0;
+  for (size_t t = 0; t + 1 < p_steps.size(); ++t) {
+    cdf += p_steps[t];
+    e += 1.0 - std::pow(cdf, 64.0);
+  }
+  return e * (step == 4 ? 18.0 : 32.0);
+}
+
 // Per launch group: the image copied to LDS (per family the layout described
 // at kSeedBlocks: 5 x 257 key starts, then the entries of blocks 0..4, keyed
 // by the 8-bit key of bases 4j..4j+3, as word pairs of two guides' other 16
 // bases per plane), padded to 16 bytes, followed by as many words naming the
 // guide behind each entry.  Fam::seed_start / seed_ent are word offsets into
-// the group's image; Fam::seed_n counts the family's entries.
+// the group's image; Fam::seed_n counts the family's entries.  Each group's
+// walk step (4 or 8 entries) is the one with the lower seed_walk_cost summed
+// over its families' blocks.
 int build_seed_tables(bgo_plan* P) {
   P->h_btab.clear();
   for (size_t gi = 0; gi < P->group_first.size(); ++gi) {
@@ -1676,6 +1719,7 @@ int build_seed_tables(bgo_plan* P) {
     const int f_end = gi + 1 < P->group_first.size() ? P->group_first[gi + 1]
                                                      : static_cast<int>(P->fams.size());
     std::vector<uint32_t> img, gix;
+    double cost4 = 0.0, cost8 = 0.0;  // the group's walks' expected VALU by step
     for (int f = f0; f < f_end; ++f) {
       Fam& F = P->fams[f];
       F.seed_start = static_cast<uint32_t>(img.size());
@@ -1713,6 +1757,10 @@ int build_seed_tables(bgo_plan* P) {
           }
           cur += static_cast<uint32_t>(b.size());
         }
+        std::vector<uint32_t> sizes(bk::kSeedKeys);
+        for (uint32_t k = 0; k < bk::kSeedKeys; ++k) sizes[k] = static_cast<uint32_t>(bucket[k].size());
+        cost4 += seed_walk_cost(sizes, 4);
+        cost8 += seed_walk_cost(sizes, 8);
       }
       F.seed_n = cur;
       img.resize((img.size() + bk::kSeedPad + 3) / 4 * 4, 0u);
@@ -1728,6 +1776,7 @@ int build_seed_tables(bgo_plan* P) {
     std::memcpy(P->h_btab.data() + off, img.data(), bytes);
     std::memcpy(P->h_btab.data() + off + bytes, gix.data(), bytes);
     P->group_img_off.push_back(off);
+    P->group_step.push_back(cost4 < cost8 ? 4 : 8);
     P->group_img_bytes.push_back(bytes);
   }
   return BGO_OK;
@@ -1925,6 +1974,7 @@ int plan_set_guides(bgo_plan* P, const bgo_guide_desc* guides, int32_t n_guides)
     }
   }
   P->group_img_off.clear();
+  P->group_step.clear();
   P->group_img_bytes.clear();
   if (P->tune.engine == 4) BGO_TRY(build_seed_tables(P));
   const size_t btab_bytes = P->h_btab.size();
@@ -2111,7 +2161,8 @@ int enqueue_scan(bgo_plan* P, int64_t pos_begin, int64_t pos_end, int32_t strand
     a.btab_bytes = static_cast<uint32_t>(seed ? P->group_img_bytes[gi] : P->h_btab.size());
     a.hit_cap = static_cast<unsigned long long>(P->hit_cap);
     const int8_t* btab = seed ? P->d_btab + P->group_img_off[gi] : P->d_btab;
-    hipLaunchKernelGGL(bk::kernel_for(nf, P->tune.engine), dim3(P->group_blocks[gi]), dim3(P->threads),
+    hipLaunchKernelGGL(bk::kernel_for(nf, P->tune.engine, seed ? P->group_step[gi] : 8),
+                       dim3(P->group_blocks[gi]), dim3(P->threads),
                        P->group_lds[gi], P->stream, a, P->d_fam + f0, P->d_g01, P->d_gk, P->d_gaux, P->d_gid,
                        P->d_gnm, P->d_score, P->d_hits, P->d_counters,
                        reinterpret_cast<unsigned int*>(P->d_counters + P->work_word) +
